@@ -1,0 +1,591 @@
+"""CPU oracle for the TD-SCHA ensemble L-application (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker. The product
+library never routes through it.
+
+Contents
+--------
+* ctypes wrappers around ``liboracle.so`` (this project's C restatement of the
+  per-image kernel, ``tdscha_oracle.c``) and, when it has been built, around
+  ``_ref/libref_lanczos.so`` (the reference's own C trio compiled from its
+  source by ``oracle/Makefile``).
+* A numpy restatement of the Python operator layer that wraps the kernel in
+  the reference ``Modules/DynamicalLanczos.py`` ("Py" below):
+
+  - Bose occupation / f_ups                 Py:144-163
+  - psi <-> symmetric matrices              Py:2827-2888 (get_Y1, get_ReA1)
+  - Wigner change of variables              Py:2572-2627, Py:2713-2822
+  - harmonic part apply_L1_FT               Py:2227-2385
+  - anharmonic pre/post transforms          Py:2891-3281 (apply_anharmonic_FT)
+  - apply_full_L (reverse / shift)          Py:3467-3570
+  - Wigner dot-product mask                 Py:5790-5839
+  - biconjugate / symmetric Lanczos run_FT  Py:5843-6372
+  - continued-fraction Green function       Py:4951-5047
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RY_TO_K = 157887.32400374097   # Py:134, CModules/LanczosFunctions.h:9
+EPSILON = 1e-12                # Py:121
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ORACLE_SO = os.path.join(_HERE, "liboracle.so")
+_REF_SO = os.path.join(_HERE, "_ref", "libref_lanczos.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_dpp = ctypes.POINTER(_dp)
+_ipp = ctypes.POINTER(_ip)
+
+
+# --------------------------------------------------------------------------
+# Symmetry description (the reference binding's sym_list / n_deg /
+# sym_basis_list / blocks_ids, odd_corr_module.c:519-601)
+# --------------------------------------------------------------------------
+@dataclass
+class SymBlocks:
+    """Mode-basis symmetry blocks.
+
+    symmetries[b]       : (N_sym, d_b, d_b) float64, row-major
+    N_degeneracy[b]     : d_b
+    degenerate_space[b] : (d_b,) int32 mode ids, contiguous and sorted
+    block_id[mu]        : block of mode mu
+    """
+    symmetries: list
+    N_degeneracy: np.ndarray
+    degenerate_space: list
+    block_id: np.ndarray
+    n_syms: int = field(init=False)
+
+    def __post_init__(self):
+        self.symmetries = [np.ascontiguousarray(s, dtype=np.float64) for s in self.symmetries]
+        self.N_degeneracy = np.ascontiguousarray(self.N_degeneracy, dtype=np.intc)
+        self.degenerate_space = [np.ascontiguousarray(d, dtype=np.intc) for d in self.degenerate_space]
+        self.block_id = np.ascontiguousarray(self.block_id, dtype=np.intc)
+        self.n_syms = int(self.symmetries[0].shape[0])
+
+    @property
+    def n_blocks(self):
+        return len(self.symmetries)
+
+    def subset(self, n_sym):
+        """The first n_sym symmetries (a bounded sample of the image set)."""
+        return SymBlocks([s[:n_sym] for s in self.symmetries], self.N_degeneracy,
+                         self.degenerate_space, self.block_id)
+
+    def dense(self, s):
+        """Dense n x n matrix of symmetry s."""
+        n = len(self.block_id)
+        out = np.zeros((n, n))
+        for b, sb in enumerate(self.symmetries):
+            idx = self.degenerate_space[b]
+            out[np.ix_(idx, idx)] = sb[s]
+        return out
+
+    @staticmethod
+    def identity(n):
+        """No symmetries: one 1x1 identity block per mode (Py:721-726)."""
+        return SymBlocks([np.ones((1, 1, 1))] * n, np.ones(n, dtype=np.intc),
+                         [np.array([i], dtype=np.intc) for i in range(n)],
+                         np.arange(n, dtype=np.intc))
+
+
+class _PtrArrays:
+    """double** / int** views kept alive for the duration of a C call."""
+
+    def __init__(self, sym: SymBlocks):
+        self._syms = sym.symmetries
+        self._degs = sym.degenerate_space
+        nb = sym.n_blocks
+        self.syms = (_dp * nb)(*[s.ctypes.data_as(_dp) for s in self._syms])
+        self.degs = (_ip * nb)(*[d.ctypes.data_as(_ip) for d in self._degs])
+        self.ndeg = sym.N_degeneracy
+        self.bid = sym.block_id
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+_lib_oracle = None
+_lib_ref = None
+
+
+def _load_oracle():
+    global _lib_oracle
+    if _lib_oracle is None:
+        if not os.path.exists(_ORACLE_SO):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        lib = ctypes.CDLL(_ORACLE_SO)
+        lib.oracle_perturb_average_sym_range.restype = ctypes.c_int
+        lib.oracle_perturb_average_sym_range.argtypes = [
+            _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+            ctypes.c_int, _dp, _dp, _dp, _dp]
+        lib.oracle_f_ups.restype = ctypes.c_double
+        lib.oracle_f_ups.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.oracle_f_psi.restype = ctypes.c_double
+        lib.oracle_f_psi.argtypes = [ctypes.c_double, ctypes.c_double]
+        _lib_oracle = lib
+    return _lib_oracle
+
+
+def ref_available():
+    return os.path.exists(_REF_SO)
+
+
+def _load_ref():
+    global _lib_ref
+    if _lib_ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libref_lanczos.so is not built")
+        lib = ctypes.CDLL(_REF_SO)
+        argt = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dpp,
+                ctypes.c_int, _ip, _ipp, _ip, _dp]
+        for name in ("get_f_average_from_Y_pert_sym_fast", "get_d2v_dR2_from_R_pert_sym_fast",
+                     "get_d2v_dR2_from_Y_pert_sym_fast"):
+            fn = getattr(lib, name)
+            fn.restype = None
+            fn.argtypes = argt
+        _lib_ref = lib
+    return _lib_ref
+
+
+F_TERM, D2V_R_TERM, D2V_Y_TERM = 1, 2, 4
+
+
+def perturb_average_sym(X, Y, w, rho, R1, Y1, T, apply_d4, sym: SymBlocks, n_threads=None,
+                        cfg_range=None, sym_range=None, what=None, norm=None, with_abs=False):
+    """Oracle (f_av, d2v) with the output contract of GetPerturbAverageSym.
+
+    cfg_range / sym_range restrict the images to a slice; `norm` overrides the
+    (N_eff, N_sym) normalisation pair (default: full-ensemble values).
+    with_abs=True also returns (f_abs, d2v_abs), the same sums taken over the
+    absolute values of every term (the rounding-error scale).
+    """
+    lib = _load_oracle()
+    X, Y, w, rho, R1, Y1 = (_c(a) for a in (X, Y, w, rho, R1, Y1))
+    N, n = X.shape
+    if n_threads is None:
+        n_threads = max(1, min(8, os.cpu_count() or 1))
+    c0, c1 = cfg_range if cfg_range is not None else (0, N)
+    s0, s1 = sym_range if sym_range is not None else (0, sym.n_syms)
+    if what is None:
+        what = F_TERM | D2V_R_TERM | (D2V_Y_TERM if apply_d4 else 0)
+    neff, nsym = norm if norm is not None else (float(np.sum(rho)), float(sym.n_syms))
+    p = _PtrArrays(sym)
+    f = np.zeros(n)
+    d2v = np.zeros((n, n))
+    fa = np.zeros(n) if with_abs else None
+    da = np.zeros((n, n)) if with_abs else None
+    rc = lib.oracle_perturb_average_sym_range(
+        X.ctypes.data_as(_dp), Y.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+        rho.ctypes.data_as(_dp), R1.ctypes.data_as(_dp), Y1.ctypes.data_as(_dp), float(T), n, N,
+        sym.n_syms, sym.n_blocks, p.syms, p.ndeg.ctypes.data_as(_ip), p.degs,
+        p.bid.ctypes.data_as(_ip), int(what), int(c0), int(c1), int(s0), int(s1), float(neff),
+        float(nsym), int(n_threads), f.ctypes.data_as(_dp), d2v.ctypes.data_as(_dp),
+        fa.ctypes.data_as(_dp) if with_abs else None, da.ctypes.data_as(_dp) if with_abs else None)
+    if rc != 0:
+        raise ValueError("oracle rejected its arguments (code %d)" % rc)
+    if with_abs:
+        return f, d2v, fa, da
+    return f, d2v
+
+
+def ref_perturb_average_sym(X, Y, w, rho, R1, Y1, T, apply_d4, sym: SymBlocks,
+                            n_configs=None, n_syms=None, quiet=True):
+    """The reference's C trio, called in the order of odd_corr_module.c:613-627.
+
+    n_configs / n_syms take the first configs / symmetries only (the arrays are
+    laid out config-major and symmetry-major, so a prefix is a valid input).
+    With quiet=True the per-call printf of LanczosFunctions.c:2139 is sent to
+    /dev/null.
+    """
+    lib = _load_ref()
+    X, Y, w, rho = (_c(a) for a in (X, Y, w, rho))
+    R1 = _c(R1).copy()
+    Y1 = _c(Y1).copy()
+    N, n = X.shape
+    if n_configs is not None:
+        N = int(n_configs)
+    nsym = sym.n_syms if n_syms is None else int(n_syms)
+    p = _PtrArrays(sym)
+    f = np.zeros(n)
+    d2v = np.zeros((n, n))
+    args_common = (X.ctypes.data_as(_dp), Y.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    tail = (float(T), n, N, rho.ctypes.data_as(_dp), p.syms, nsym, p.ndeg.ctypes.data_as(_ip),
+            p.degs, p.bid.ctypes.data_as(_ip))
+    saved = None
+    if quiet:
+        import sys
+        sys.stdout.flush()
+        saved = os.dup(1)
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(devnull, 1)
+        os.close(devnull)
+    try:
+        lib.get_f_average_from_Y_pert_sym_fast(*args_common, Y1.ctypes.data_as(_dp), *tail,
+                                               f.ctypes.data_as(_dp))
+        lib.get_d2v_dR2_from_R_pert_sym_fast(*args_common, R1.ctypes.data_as(_dp), *tail,
+                                             d2v.ctypes.data_as(_dp))
+        if apply_d4:
+            lib.get_d2v_dR2_from_Y_pert_sym_fast(*args_common, Y1.ctypes.data_as(_dp), *tail,
+                                                 d2v.ctypes.data_as(_dp))
+    finally:
+        if saved is not None:
+            import ctypes.util
+            libc = ctypes.CDLL(ctypes.util.find_library("c"))
+            libc.fflush(None)
+            os.dup2(saved, 1)
+            os.close(saved)
+    return f, d2v
+
+
+# --------------------------------------------------------------------------
+# Python operator layer (restated from Modules/DynamicalLanczos.py)
+# --------------------------------------------------------------------------
+def bose_occupation(w, T):
+    """Py:157-163 (threshold T > 0)."""
+    w = np.asarray(w, dtype=np.float64)
+    if T > 0:
+        return 1.0 / (np.exp(w * RY_TO_K / T) - 1.0)
+    return np.zeros_like(w)
+
+
+def f_ups(w, T):
+    """Py:144-155: 2w / (1 + 2n)."""
+    return 2 * np.asarray(w) / (1 + 2 * bose_occupation(w, T))
+
+
+def pair_indices(n):
+    """Independent (row b, col a >= b) pairs in psi order (Py:2280-2286).
+
+    Returns (i_row, i_col): the upper triangle, row-major.
+    """
+    iu = np.triu_indices(n)
+    return iu[0], iu[1]
+
+
+def psi_len(n):
+    return n + n * (n + 1)
+
+
+def unpack_sym(vec, n, half_off_diagonal=False):
+    """Upper-triangle row-major vector -> symmetric matrix (Py:2827-2858)."""
+    M = np.zeros((n, n))
+    iu = np.triu_indices(n)
+    M[iu] = vec
+    M = M + np.triu(M, 1).T
+    if half_off_diagonal:
+        M = M * 0.5
+        M[np.diag_indices(n)] *= 2
+    return M
+
+
+def pack_upper(M):
+    """Matrix -> upper-triangle row-major vector (Py:3250-3262)."""
+    return np.asarray(M)[np.triu_indices(M.shape[0])].copy()
+
+
+def _pair_freqs(w, T):
+    """w_a, w_b, n_a, n_b per independent pair as apply_L1_FT builds them (Py:2280-2301)."""
+    i_b, i_a = pair_indices(len(w))
+    w_a, w_b = w[i_a], w[i_b]
+    if T > 0:
+        n_a = 1 / (np.exp(w_a / np.double(T / RY_TO_K)) - 1)
+        n_b = 1 / (np.exp(w_b / np.double(T / RY_TO_K)) - 1)
+    else:
+        n_a = np.zeros_like(w_a)
+        n_b = np.zeros_like(w_b)
+    return w_a, w_b, n_a, n_b
+
+
+def apply_L1_FT(psi, w, T, transpose=False, use_wigner=False, ignore_harmonic=False):
+    """Harmonic propagator, Py:2227-2385."""
+    n = len(w)
+    out = np.zeros_like(psi)
+    if ignore_harmonic:
+        return out
+    w_a, w_b, n_a, n_b = _pair_freqs(w, T)
+    nw2 = len(w_a)
+    sY, sA = n, n + nw2
+    assert len(psi) == sA + nw2
+    if not use_wigner:
+        out[:n] = (psi[:n] * w) * w
+        X_ab = -w_a**2 - w_b**2 - (2 * w_a * w_b) / ((2 * n_a + 1) * (2 * n_b + 1))
+        out[sY:sA] = -X_ab * psi[sY:sA]
+        Y_ab = -(8 * w_a * w_b) / ((2 * n_a + 1) * (2 * n_b + 1))
+        if not transpose:
+            out[sY:sA] += -Y_ab * psi[sA:]
+        else:
+            out[sA:] += -Y_ab * psi[sY:sA]
+        X1_ab = -(2 * n_a * n_b + n_a + n_b) * (2 * n_a * n_b + n_a + n_b + 1) * (2 * w_a * w_b) / (
+            (2 * n_a + 1) * (2 * n_b + 1))
+        if not transpose:
+            out[sA:] += -X1_ab * psi[sY:sA]
+        else:
+            out[sY:sA] += -X1_ab * psi[sA:]
+        Y1_ab = -w_a**2 - w_b**2 + (2 * w_a * w_b) / ((2 * n_a + 1) * (2 * n_b + 1))
+        out[sA:] += -Y1_ab * psi[sA:]
+    else:
+        out[:n] = -(psi[:n] * w) * w
+        out[sY:sA] = -(w_a**2 + w_b**2 - 2.0 * w_a * w_b) * psi[sY:sA]
+        out[sA:] = -(w_a**2 + w_b**2 + 2.0 * w_a * w_b) * psi[sA:]
+    return out
+
+
+def chi_minus_plus(w, T):
+    """Py:2572-2627 (threshold T > EPSILON)."""
+    n = len(w)
+    W = np.tile(w, (n, 1))
+    nn = np.zeros((n, n))
+    if T > EPSILON:
+        nn = 1.0 / (np.exp(W * RY_TO_K / T) - 1.0)
+    chi_m = (W - W.T) * (nn - nn.T) / (2.0 * W * W.T)
+    chi_p = (W + W.T) * (1 + nn + nn.T) / (2.0 * W * W.T)
+    return chi_m, chi_p
+
+
+def wigner_alpha1(psi, w, T):
+    """Upsilon^(1) = alpha^(1) from (a', b'), Py:2713-2800 (get_alpha=True)."""
+    n = len(w)
+    nw2 = n * (n + 1) // 2
+    a_all = psi[n:n + nw2]
+    b_all = psi[n + nw2:]
+    w_a, w_b, n_a, n_b = _pair_freqs(w, T)
+    Xf = (1 + 2 * n_a) * (1 + 2 * n_b) / 8
+    w2_on_X = (w_a * w_b) / Xf
+    chi_m = ((w_a - w_b) * (n_a - n_b)) / (2 * w_a * w_b)
+    chi_p = ((w_a + w_b) * (1 + n_a + n_b)) / (2 * w_a * w_b)
+    new_a = w2_on_X * np.sqrt(-0.5 * chi_m) * a_all
+    new_b = w2_on_X * np.sqrt(+0.5 * chi_p) * b_all
+    return unpack_sym(new_a - new_b, n)
+
+
+def anharmonic_pre(psi, w, T, transpose=False, use_wigner=False):
+    """Py:2946-2993: the (R1, Y1) handed to the ensemble kernel."""
+    n = len(w)
+    nw2 = n * (n + 1) // 2
+    R1 = psi[:n].copy()
+    if use_wigner:
+        return R1, wigner_alpha1(psi, w, T)
+    Y1 = unpack_sym(psi[n:n + nw2], n, half_off_diagonal=transpose)
+    if transpose:
+        n_mu = bose_occupation(w, T) if T > EPSILON else np.zeros(n)
+        Y_w = 2 * w / (2 * n_mu + 1)
+        ReA_w = 2 * w * n_mu * (n_mu + 1) / (2 * n_mu + 1)
+        ReA1 = unpack_sym(psi[n + nw2:], n, half_off_diagonal=True)
+        coeff_Y = 2 * (np.einsum("a,b,b->ab", Y_w, Y_w, Y_w) + np.einsum("a,a,b->ab", Y_w, Y_w, Y_w))
+        coeff_RA = 2 * (np.einsum("a,b,b->ab", Y_w, ReA_w, Y_w) + np.einsum("a,a,b->ab", Y_w, ReA_w, Y_w))
+        Y1 = Y1 * coeff_Y + ReA1 * coeff_RA
+    return R1, Y1
+
+
+def anharmonic_post(f_av, d2v, w, T, transpose=False, use_wigner=False):
+    """Py:3200-3281: kernel outputs -> psi' (sign included)."""
+    n = len(w)
+    if use_wigner:
+        chi_m, chi_p = chi_minus_plus(w, T)
+        pert_Y = np.sqrt(-0.5 * chi_m) * d2v
+        pert_RA = -np.sqrt(0.5 * chi_p) * d2v
+        sign = 1.0
+    else:
+        n_mu = 1.0 / (np.exp(w * RY_TO_K / T) - 1.0) if T > EPSILON else np.zeros(n)
+        Y_w = 2 * w / (2 * n_mu + 1)
+        ReA_w = 2 * w * n_mu * (n_mu + 1) / (2 * n_mu + 1)
+        if not transpose:
+            pert_Y = d2v * Y_w[:, None] + d2v * Y_w[None, :]
+            pert_RA = d2v * ReA_w[:, None] + d2v * ReA_w[None, :]
+        else:
+            Y_inv = 1 / Y_w
+            pert_Y = 0.5 * np.einsum("a,ab,b->ab", Y_inv, d2v, Y_inv)
+            pert_RA = np.zeros_like(pert_Y)
+            mask = np.full(pert_Y.shape, 2.0)
+            np.fill_diagonal(mask, 1)
+            pert_Y *= mask
+            pert_RA *= mask
+        sign = -1.0
+    out = np.concatenate([f_av, pack_upper(pert_Y), pack_upper(pert_RA)])
+    return sign * out
+
+
+@dataclass
+class Ensemble:
+    """The projected ensemble the operator is built on (Py:380-442)."""
+    X: np.ndarray
+    Y: np.ndarray
+    w: np.ndarray
+    rho: np.ndarray
+    T: float
+    sym: SymBlocks
+
+
+def apply_anharmonic_FT(ens: Ensemble, psi, transpose=False, use_wigner=False, ignore_v4=False,
+                        kernel=None):
+    """Py:2891-3281. `kernel(R1, Y1, apply_d4) -> (f, d2v)` defaults to the C oracle."""
+    R1, Y1 = anharmonic_pre(psi, ens.w, ens.T, transpose, use_wigner)
+    apply_d4 = 0 if ignore_v4 else 1
+    if kernel is None:
+        f_av, d2v = perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, apply_d4, ens.sym)
+    else:
+        f_av, d2v = kernel(R1, Y1, apply_d4)
+    return anharmonic_post(f_av, d2v, ens.w, ens.T, transpose, use_wigner)
+
+
+def apply_full_L(ens: Ensemble, psi, transpose=False, use_wigner=False, ignore_v3=False,
+                 ignore_v4=False, ignore_harmonic=False, reverse_L=False, shift_value=0.0,
+                 kernel=None):
+    """Py:3467-3570 (finite-temperature branch, force_FT=True default)."""
+    out = apply_L1_FT(psi, ens.w, ens.T, transpose, use_wigner, ignore_harmonic)
+    if not ignore_v3:
+        out = out + apply_anharmonic_FT(ens, psi, transpose, use_wigner, ignore_v4, kernel)
+    if reverse_L:
+        out *= -1
+    out += shift_value * psi
+    return out
+
+
+def mask_dot_wigner(n):
+    """Py:5790-5839: weight 2 on the off-diagonal pair entries."""
+    mask = np.ones(psi_len(n))
+    i_b, i_a = pair_indices(n)
+    nw2 = len(i_a)
+    off = i_b < i_a
+    mask[n:n + nw2][off] = 2
+    mask[n + nw2:][off] = 2
+    return mask
+
+
+@dataclass
+class LanczosState:
+    a: list = field(default_factory=list)
+    b: list = field(default_factory=list)
+    c: list = field(default_factory=list)
+    Q: list = field(default_factory=list)
+    P: list = field(default_factory=list)
+    s_norm: list = field(default_factory=list)
+
+
+def run_FT(matvec, rmatvec, psi0, n_iter, run_simm=False, mask=None, n_rep_orth=0, n_ortho=10,
+           state: LanczosState | None = None):
+    """Biconjugate / symmetric Lanczos of Py:5843-6372 (verbatim recurrence).
+
+    matvec(q) = L q, rmatvec(p) = L^T p. Returns the LanczosState.
+    The Gram-Schmidt block follows Py:6245-6317 as written (including its
+    normalisation of p by p.p and its ortho_p accumulator).
+    """
+    st = state if state is not None else LanczosState()
+    dot = (lambda u, v: u.dot(v * mask)) if run_simm else (lambda u, v: u.dot(v))
+    if len(st.a) == 0:
+        first = psi0 / np.sqrt(dot(psi0, psi0))
+        st.Q, st.P, st.s_norm = [first], [first], [1.0]
+    psi_q, psi_p = st.Q[-1], st.P[-1]
+    next_converged = False
+    for _ in range(n_iter):
+        L_q = matvec(psi_q)
+        p_L = L_q.copy() if run_simm else rmatvec(psi_p)
+        c_old = st.c[-1] if len(st.c) > 0 else 1
+        p_norm = st.s_norm[-1] / c_old
+        a_coeff = dot(psi_p, L_q) * p_norm
+        if np.isnan(a_coeff):
+            raise ValueError("Invalid value encountered during the Lanczos.")
+        rk = L_q - a_coeff * psi_q
+        if len(st.Q) > 1:
+            rk -= st.c[-1] * st.Q[-2]
+        sk = p_L - a_coeff * psi_p
+        if len(st.P) > 1:
+            old_p_norm = st.s_norm[-2] if len(st.c) < 2 else st.s_norm[-2] / st.c[-2]
+            sk -= st.b[-1] * st.P[-2] * (old_p_norm / p_norm)
+        s_norm = np.sqrt(dot(sk, sk))
+        sk_tilde = sk / s_norm
+        s_norm *= p_norm
+        b_coeff = np.sqrt(dot(rk, rk))
+        c_coeff = dot(sk_tilde, rk / b_coeff) * s_norm
+        st.a.append(a_coeff)
+        if np.abs(b_coeff) < EPSILON or next_converged:
+            break
+        if np.abs(c_coeff) < EPSILON:
+            break
+        new_q = rk / b_coeff
+        new_p = sk_tilde.copy()
+        for _k in range(n_rep_orth):
+            ortho_q = 0.0
+            ortho_p = 0.0
+            start = 0 if n_ortho is None else max(0, len(st.P) - n_ortho)
+            for j in range(start, len(st.P)):
+                coeff1 = dot(st.P[j], new_q)
+                coeff2 = dot(st.Q[j], new_p)
+                new_q = new_q - coeff1 * st.P[j]
+                new_p = new_p - coeff2 * st.Q[j]
+                ortho_q += np.abs(coeff1)
+                ortho_p += np.abs(ortho_p)
+            normq = np.sqrt(dot(new_q, new_q))
+            if normq < EPSILON:
+                next_converged = True
+            new_q = new_q / normq
+            normp = dot(new_p, new_p)
+            if np.abs(normp) < EPSILON:
+                next_converged = True
+            new_p = new_p / normp
+            s_norm = c_coeff / dot(new_p, new_q)
+            if ortho_p < EPSILON and ortho_q < EPSILON:
+                break
+        st.Q.append(new_q)
+        st.P.append(new_p)
+        psi_q, psi_p = new_q.copy(), new_p.copy()
+        st.b.append(b_coeff)
+        st.c.append(c_coeff)
+        st.s_norm.append(s_norm)
+    return st
+
+
+def green_function_continued_fraction(a, b, c, w_array, use_terminator=True, last_average=1,
+                                      smearing=0.0, use_wigner=False, reverse_L=False,
+                                      shift_value=0.0, perturbation_modulus=1.0):
+    """Py:4951-5047."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    c = np.asarray(c, dtype=np.float64)
+    w_array = np.asarray(w_array, dtype=np.float64)
+    n_iters = len(a)
+    sign = -1 if reverse_L else 1
+    gf = np.zeros(w_array.shape, dtype=np.complex128)
+    if use_terminator:
+        a_av = np.mean(a[-last_average:])
+        b_av = np.mean(b[-last_average:])
+        c_av = b_av
+        if len(c) == len(b):
+            c_av = np.mean(c[-last_average:])
+        aa = a_av * sign - sign * shift_value
+        bb = b_av * sign
+        cc = c_av * sign
+        if not use_wigner:
+            gf[:] = (aa - w_array**2 - np.sqrt((aa - w_array**2)**2 - 4 * bb * cc + 0j)) / (2 * bb * cc)
+        else:
+            gf[:] = (aa + w_array**2 + np.sqrt((aa + w_array**2)**2 - 4 * bb * cc + 0j)) / (2 * bb * cc)
+    else:
+        aa = a[-1] * sign - sign * shift_value
+        if not use_wigner:
+            gf[:] = 1 / (aa - w_array**2 + 2j * w_array * smearing)
+        else:
+            gf[:] = 1 / (aa + w_array**2 + 2j * w_array * smearing)
+    for i in range(n_iters - 2, -1, -1):
+        aa = a[i] * sign - sign * shift_value
+        bb = b[i] * sign
+        cc = bb
+        if len(c) == len(b):
+            cc = c[i] * sign
+        if not use_wigner:
+            gf = 1.0 / (aa - w_array**2 + 2j * w_array * smearing - bb * cc * gf)
+        else:
+            gf = 1.0 / (aa + w_array**2 + 2j * w_array * smearing - bb * cc * gf)
+    if not use_wigner:
+        return gf * perturbation_modulus
+    return (-np.real(gf) + 1j * np.imag(gf)) * perturbation_modulus
