@@ -1,0 +1,157 @@
+/*
+ * libtdscha_hip: MI355X (gfx950) implementation of the TD-SCHA ensemble
+ * L-application, the symmetry-expanded weighted ensemble average that every
+ * TD-SCHA Lanczos step calls.
+ *
+ * C ABI only: plain pointers and sizes. Every function returning int returns
+ * 0 on success and a non-zero code on failure; tdscha_last_error() then holds
+ * a message (thread-local). The reference signals bad input by printing and
+ * calling exit() (CModules/odd_corr_module.c:535-547); the prefixed entry points
+ * here return an error instead. The unprefixed drop-in symbols keep the
+ * reference's void signatures and therefore keep its behaviour on failure
+ * (message on stderr, exit(EXIT_FAILURE)).
+ *
+ * Units and layouts follow the reference exactly:
+ *   X, Y  (n_configs, n_modes) row-major, mode basis (Bohr sqrt(m) / Ry/Bohr/sqrt(m))
+ *   w     (n_modes) SCHA frequencies [Ry];  T in K;  w_is / rho (n_configs)
+ *   symmetries[b] (N_sym, d_b, d_b) row-major acting on the modes
+ *   degenerate_space[b][0..d_b-1], which must be contiguous and sorted
+ *   (the reference indexes rows as mu - degenerate_space[b][0],
+ *   LanczosFunctions.c:2162), blocks_ids[mu] = block of mode mu.
+ * Frequencies must be equal (to 1e-8 relative) inside a block, as they are for
+ * blocks of degenerate modes; the device path uses this to rotate the n x n
+ * operands instead of building every image.
+ *
+ * Thread safety: calls on different ensembles may run concurrently; calls on
+ * one ensemble share its workspace and must not overlap.
+ */
+#ifndef TDSCHA_HIP_H
+#define TDSCHA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDSCHA_ABI_VERSION 1
+
+/* Arithmetic of the two ensemble contractions. */
+#define TDSCHA_PREC_FP64 0   /* FP64 MFMA, the plain path */
+#define TDSCHA_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation in FP64-flushed chunks */
+#define TDSCHA_PREC_BF16X3 2 /* hi*hi + hi*lo + lo*hi BF16 MFMA, FP32 accumulation */
+
+/* Terms of the perturbed averages (bit mask). */
+#define TDSCHA_TERM_F 1     /* <f>^(1) from Y1            (LanczosFunctions.c:2082) */
+#define TDSCHA_TERM_D2V_R 2 /* <d2V/dR2>^(1), D3 from R1   (LanczosFunctions.c:2257) */
+#define TDSCHA_TERM_D2V_Y 4 /* <d2V/dR2>^(1), D4 from Y1   (LanczosFunctions.c:2615) */
+
+/* Flags of tdscha_apply_full_L (Lanczos attributes, Modules/DynamicalLanczos.py:235-290). */
+#define TDSCHA_OP_TRANSPOSE 1
+#define TDSCHA_OP_WIGNER 2
+#define TDSCHA_OP_IGNORE_V3 4
+#define TDSCHA_OP_IGNORE_V4 8
+#define TDSCHA_OP_IGNORE_HARMONIC 16
+#define TDSCHA_OP_REVERSE 32
+
+typedef struct tdscha_ensemble tdscha_ensemble;
+
+const char *tdscha_last_error(void);
+int tdscha_abi_version(void);
+
+/* ---- Drop-in replacements of the reference's C trio (host pointers) ---------------- */
+
+/* Replaces get_f_average_from_Y_pert_sym_fast, CModules/LanczosFunctions.h:264-266.
+ * f_average (n_modes) is overwritten. */
+void get_f_average_from_Y_pert_sym_fast(double *X, double *Y, double *w, double *Y1, double T,
+                                        int n_modes, int n_configs, double *w_is,
+                                        double **symmetries, int N_sym, int *N_degeneracy,
+                                        int **degenerate_space, int *blocks_ids,
+                                        double *f_average);
+/* Replaces get_d2v_dR2_from_R_pert_sym_fast, CModules/LanczosFunctions.h:267-270.
+ * d2v_dR2 (n_modes x n_modes) is overwritten. */
+void get_d2v_dR2_from_R_pert_sym_fast(double *X, double *Y, double *w, double *R1, double T,
+                                      int n_modes, int n_configs, double *w_is,
+                                      double **symmetries, int N_sym, int *N_degeneracy,
+                                      int **degenerate_space, int *blocks_ids, double *d2v_dR2);
+/* Replaces get_d2v_dR2_from_Y_pert_sym_fast, CModules/LanczosFunctions.h:271-273.
+ * The D4 part is ADDED to d2v_dR2_out, as in the reference. */
+void get_d2v_dR2_from_Y_pert_sym_fast(double *X, double *Y, double *w, double *Y1, double T,
+                                      int n_modes, int n_configs, double *w_is,
+                                      double **symmetries, int N_sym, int *N_degeneracy,
+                                      int **degenerate_space, int *blocks_ids,
+                                      double *d2v_dR2_out);
+
+/* Error-returning forms of the three above (same arguments). */
+int tdscha_get_f_average_from_Y_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                              const double *Y1, double T, int n_modes,
+                                              int n_configs, const double *w_is,
+                                              const double *const *symmetries, int N_sym,
+                                              const int *N_degeneracy,
+                                              const int *const *degenerate_space,
+                                              const int *blocks_ids, double *f_average);
+int tdscha_get_d2v_dR2_from_R_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                            const double *R1, double T, int n_modes,
+                                            int n_configs, const double *w_is,
+                                            const double *const *symmetries, int N_sym,
+                                            const int *N_degeneracy,
+                                            const int *const *degenerate_space,
+                                            const int *blocks_ids, double *d2v_dR2);
+int tdscha_get_d2v_dR2_from_Y_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                            const double *Y1, double T, int n_modes,
+                                            int n_configs, const double *w_is,
+                                            const double *const *symmetries, int N_sym,
+                                            const int *N_degeneracy,
+                                            const int *const *degenerate_space,
+                                            const int *blocks_ids, double *d2v_dR2_out);
+
+/* Replaces sscha_HP_odd.GetPerturbAverageSym, CModules/odd_corr_module.c:491-634
+ * (argument order of its "OOOOOOdiiOOOOOO" format, with the list lengths and array
+ * dimensions made explicit). force_out (n) is overwritten; d2v_out (n x n) is
+ * overwritten with the D3 part, then the D4 part is added when apply_D4 is 1. */
+int tdscha_get_perturb_average_sym(const double *X, const double *Y, const double *w,
+                                   const double *rho, const double *R1, const double *Y1,
+                                   double T, int apply_D4, int N_syms, int n_blocks,
+                                   const double *const *sym_list, const int *n_deg,
+                                   const int *const *sym_basis_list, const int *blocks_ids,
+                                   int N_configs, int N_modes, double *force_out,
+                                   double *d2v_out);
+
+/* ---- Device-resident ensemble (the hot path of a Lanczos run) ------------------------
+ * No counterpart in the reference, which re-reads X, Y and the symmetries on every call
+ * (CModules/odd_corr_module.c:491-634). The ensemble is uploaded once; each apply then
+ * reads only device memory. */
+
+/* Host inputs are copied to device `device`. precisions: bit mask of (1 << TDSCHA_PREC_*)
+ * whose operand copies to build (FP64 is always built). s_batch: symmetries processed
+ * per pass (0 = automatic). */
+int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, const double *Y,
+                           const double *w, const double *rho, double T, int n_modes,
+                           int n_configs, int N_sym, int n_blocks,
+                           const double *const *symmetries, const int *N_degeneracy,
+                           const int *const *degenerate_space, const int *blocks_ids,
+                           int precisions, int s_batch);
+int tdscha_ensemble_destroy(tdscha_ensemble *ens);
+/* n_modes, n_configs, N_sym, s_batch, padded modes, padded configs */
+int tdscha_ensemble_dims(const tdscha_ensemble *ens, int *dims6);
+/* Device bytes held by the ensemble (operands + workspace). */
+long long tdscha_ensemble_bytes(const tdscha_ensemble *ens);
+
+/* The perturbed averages on device pointers, enqueued on `stream` (a hipStream_t, NULL =
+ * default stream). R1 (n), Y1 (n x n) in; f (n), d2v (n x n) out. terms selects the
+ * reference terms; f is overwritten when TERM_F is set; d2v is overwritten, or added to
+ * when d2v_accumulate is 1. Y1 need not be symmetric: its symmetric part is what the
+ * reference's formulas use. */
+int tdscha_ensemble_apply(tdscha_ensemble *ens, int precision, int terms, const double *R1,
+                          const double *Y1, double *f, double *d2v, int d2v_accumulate,
+                          void *stream);
+
+/* Full TD-SCHA L on a device vector psi of length n + n (n + 1), as
+ * Lanczos.apply_full_L (Modules/DynamicalLanczos.py:3467-3570): harmonic part,
+ * anharmonic part through the ensemble, reverse and shift. out must not alias psi. */
+int tdscha_apply_full_L(tdscha_ensemble *ens, int precision, int flags, double shift,
+                        const double *psi, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDSCHA_HIP_H */

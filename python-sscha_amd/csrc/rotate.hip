@@ -1,0 +1,282 @@
+// Symmetry rotations and the small per-pass kernels of the ensemble L-application.
+//
+// The reference builds every image u = S_s x_i explicitly (LanczosFunctions.c:2158-2170)
+// and evaluates n^2 quadratic forms on it. Here the symmetry acts on the n x n operands
+// instead, because S_s is block diagonal with blocks of degenerate modes and f_ups/f_psi
+// are constant on each block (so they commute with S_s):
+//   u^T Y1 u     = x^T (S^T Y1 S) x                       (rotate Y1 once per symmetry)
+//   sum_i c ut f^T = S (X~^T diag(c) Y) S^T               (rotate the n x n result back)
+// Rotations cost O(n^2 d_b) per symmetry, against O(N n^2) for the contractions.
+#include "tdscha_internal.h"
+
+namespace tdscha {
+
+namespace {
+
+__global__ void k_prep(const double *__restrict__ R1, const double *__restrict__ Y1, double *Ysym,
+                       double *R1p, int n, int npd) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int a = blockIdx.y;
+  if (b >= npd) return;
+  double v = 0.0;
+  if (Y1 && a < n && b < n) v = 0.5 * (Y1[(size_t)a * n + b] + Y1[(size_t)b * n + a]);
+  Ysym[(size_t)a * npd + b] = v;
+  if (a == 0) R1p[b] = (R1 && b < n) ? R1[b] : 0.0;
+}
+
+// out[s][k][b] = sum_l M[k][off_B + l] * Sel_B(l, b'),  M = in (+ in^T if symmetrize)
+// Sel_B(l, b') = S_B[l][b'] (transS = 0) or S_B[b'][l] (transS = 1).
+__global__ void k_rot_right(const double *__restrict__ in, long in_stride_s, int symmetrize,
+                            double *__restrict__ out, const double *__restrict__ S,
+                            const ModeMeta *__restrict__ meta, int n, int npd, int sym_stride,
+                            int s0, int transS) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  int sl = blockIdx.z;
+  if (b >= npd) return;
+  double acc = 0.0;
+  if (b < n && k < n) {
+    ModeMeta mb = meta[b];
+    const double *Sb = S + (size_t)(s0 + sl) * sym_stride + mb.sym_off;
+    const double *M = in + (size_t)sl * in_stride_s;
+    const double *row = M + (size_t)k * npd + mb.blk_off;
+    const int bp = b - mb.blk_off, d = mb.blk_dim;
+    for (int l = 0; l < d; ++l) {
+      double v = row[l];
+      if (symmetrize) v += M[(size_t)(mb.blk_off + l) * npd + k];
+      double sv = transS ? Sb[bp * d + l] : Sb[l * d + bp];
+      acc = fma(v, sv, acc);
+    }
+  }
+  out[(size_t)sl * npd * npd + (size_t)k * npd + b] = acc;
+}
+
+// out[s][a][b] (or out[a][b] += sum over s) = sum_k Sel_A(a', k) * in[s][off_A + k][b]
+// Sel_A(a', k) = S_A[a'][k] (transS = 0) or S_A[k][a'] (transS = 1).
+__global__ void k_rot_left(const double *__restrict__ in, double *__restrict__ out,
+                           const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
+                           int npd, int sym_stride, int s0, int sb, int transS, int accumulate) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int a = blockIdx.y;
+  if (b >= npd) return;
+  if (accumulate) {
+    if (a >= n || b >= n) return;
+    ModeMeta ma = meta[a];
+    const int ap = a - ma.blk_off, d = ma.blk_dim;
+    double acc = out[(size_t)a * npd + b];
+    for (int sl = 0; sl < sb; ++sl) {
+      const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+      const double *M = in + (size_t)sl * npd * npd + (size_t)ma.blk_off * npd + b;
+      double part = 0.0;
+      for (int k = 0; k < d; ++k) {
+        double sv = transS ? Sa[k * d + ap] : Sa[ap * d + k];
+        part = fma(sv, M[(size_t)k * npd], part);
+      }
+      acc += part;
+    }
+    out[(size_t)a * npd + b] = acc;
+  } else {
+    int sl = blockIdx.z;
+    double acc = 0.0;
+    if (a < n && b < n) {
+      ModeMeta ma = meta[a];
+      const int ap = a - ma.blk_off, d = ma.blk_dim;
+      const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+      const double *M = in + (size_t)sl * npd * npd + (size_t)ma.blk_off * npd + b;
+      for (int k = 0; k < d; ++k) {
+        double sv = transS ? Sa[k * d + ap] : Sa[ap * d + k];
+        acc = fma(sv, M[(size_t)k * npd], acc);
+      }
+    }
+    out[(size_t)sl * npd * npd + (size_t)a * npd + b] = acc;
+  }
+}
+
+// r_s = S_s^T R1;  G[s][0][a] = f_ups[a] r_s[a],  G[s][1][a] = r_s[a] / f_psi[a]
+__global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G,
+                          const double *__restrict__ S, const ModeMeta *__restrict__ meta,
+                          const double *__restrict__ fups, const double *__restrict__ fpsi, int n,
+                          int npd, int sym_stride, int s0) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  int sl = blockIdx.y;
+  if (a >= npd) return;
+  double g1 = 0.0, g2 = 0.0;
+  if (a < n) {
+    ModeMeta ma = meta[a];
+    const int ap = a - ma.blk_off, d = ma.blk_dim;
+    const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+    double r = 0.0;
+    for (int k = 0; k < d; ++k) r = fma(Sa[k * d + ap], R1p[ma.blk_off + k], r);
+    g1 = fups[a] * r;
+    g2 = r / fpsi[a];
+  }
+  G[(size_t)sl * 2 * npd + a] = g1;
+  G[(size_t)sl * 2 * npd + npd + a] = g2;
+}
+
+// V[s][j] = sum_i YT[j][i] e1[s][i] + XtT[j][i] e2[s][i]
+// Block: 4 waves x 4 rows each = 16 rows j; lanes stride over configs i.
+constexpr int kFavRowsPerWave = 4;
+constexpr int kFavMaxS = 16;
+__global__ __launch_bounds__(256) void k_fav(const double *__restrict__ YT,
+                                             const double *__restrict__ XtT,
+                                             const double *__restrict__ W, double *__restrict__ V,
+                                             int Np, int npd, int s_first, int s_count) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j0 = (blockIdx.x * 4 + wave) * kFavRowsPerWave;
+  double acc[kFavRowsPerWave][kFavMaxS];
+#pragma unroll
+  for (int r = 0; r < kFavRowsPerWave; ++r)
+#pragma unroll
+    for (int q = 0; q < kFavMaxS; ++q) acc[r][q] = 0.0;
+  for (int i = lane; i < Np; i += 64) {
+    double yv[kFavRowsPerWave], xv[kFavRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kFavRowsPerWave; ++r) {
+      yv[r] = YT[(size_t)(j0 + r) * Np + i];
+      xv[r] = XtT[(size_t)(j0 + r) * Np + i];
+    }
+#pragma unroll
+    for (int q = 0; q < kFavMaxS; ++q) {
+      if (q < s_count) {
+        const double *Ws = W + (size_t)(s_first + q) * 4 * Np;
+        double e1 = Ws[2 * (size_t)Np + i], e2 = Ws[3 * (size_t)Np + i];
+#pragma unroll
+        for (int r = 0; r < kFavRowsPerWave; ++r) acc[r][q] = fma(yv[r], e1, fma(xv[r], e2, acc[r][q]));
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kFavRowsPerWave; ++r)
+#pragma unroll
+    for (int q = 0; q < kFavMaxS; ++q) {
+      double v = acc[r][q];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0 && q < s_count) V[(size_t)(s_first + q) * npd + j0 + r] = v;
+    }
+}
+
+// f_acc[a] += sum_s sum_k S_s[a'][k] V[s][off_A + k]
+__global__ void k_fav_rot_acc(const double *__restrict__ V, double *__restrict__ f_acc,
+                              const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
+                              int npd, int sym_stride, int s0, int sb) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  ModeMeta ma = meta[a];
+  const int ap = a - ma.blk_off, d = ma.blk_dim;
+  double acc = f_acc[a];
+  for (int sl = 0; sl < sb; ++sl) {
+    const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off + ap * d;
+    const double *v = V + (size_t)sl * npd + ma.blk_off;
+    double part = 0.0;
+    for (int k = 0; k < d; ++k) part = fma(Sa[k], v[k], part);
+    acc += part;
+  }
+  f_acc[a] = acc;
+}
+
+__global__ void k_zero(double *p, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = 0.0;
+}
+
+__global__ void k_finalize(const double *__restrict__ f_acc, const double *__restrict__ d2v_acc,
+                           double *f_out, double *d2v_out, int n, int npd, double inv_norm, int do_f,
+                           int do_d, int d_accumulate) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  int a = blockIdx.y;
+  if (b >= n) return;
+  if (do_d) {
+    double v = d2v_acc[(size_t)a * npd + b] * inv_norm;
+    size_t o = (size_t)a * n + b;
+    d2v_out[o] = d_accumulate ? d2v_out[o] + v : v;
+  }
+  if (do_f && a == 0) f_out[b] = f_acc[b] * inv_norm;
+}
+
+// Ys (fp64) -> bf16 (hi, lo) planes or fp32, same [sb x npd x npd] layout.
+__global__ void k_split_ys(const double *__restrict__ Ys, void *out, size_t count, int prec) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) {
+    double v = Ys[i];
+    if (prec == kBF16x3) {
+      __hip_bfloat16 *hi = (__hip_bfloat16 *)out;
+      __hip_bfloat16 *lo = hi + count;
+      float f = (float)v;
+      __hip_bfloat16 h = __float2bfloat16(f);
+      hi[i] = h;
+      lo[i] = __float2bfloat16(f - __bfloat162float(h));
+    } else {
+      ((float *)out)[i] = (float)v;
+    }
+  }
+}
+
+inline dim3 grid1(size_t count, int block) {
+  size_t g = (count + block - 1) / block;
+  if (g > 65535u * 16u) g = 65535u * 16u;
+  return dim3((unsigned)g);
+}
+
+}  // namespace
+
+void launch_prep_inputs(const Ensemble &e, const double *R1, const double *Y1, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, e.npd);
+  hipLaunchKernelGGL(k_prep, grid, dim3(256), 0, st, R1, Y1, e.Ysym, e.R1p, e.n, e.npd);
+}
+
+void launch_rot_right(const Ensemble &e, const double *in, long in_stride_s, bool symmetrize,
+                      double *out, int s0, int sb, bool transS, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, e.npd, sb);
+  hipLaunchKernelGGL(k_rot_right, grid, dim3(256), 0, st, in, in_stride_s, (int)symmetrize, out, e.S,
+                     e.meta, e.n, e.npd, e.sym_stride, s0, (int)transS);
+}
+
+void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, int sb, bool transS,
+                     bool accumulate, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, e.npd, accumulate ? 1 : sb);
+  hipLaunchKernelGGL(k_rot_left, grid, dim3(256), 0, st, in, out, e.S, e.meta, e.n, e.npd,
+                     e.sym_stride, s0, sb, (int)transS, (int)accumulate);
+}
+
+void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, sb);
+  hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, e.R1p, e.G, e.S, e.meta, e.fups, e.fpsi,
+                     e.n, e.npd, e.sym_stride, s0);
+}
+
+void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
+  dim3 grid(e.npd / (4 * kFavRowsPerWave));
+  for (int q0 = 0; q0 < sb; q0 += kFavMaxS) {
+    int cnt = sb - q0 < kFavMaxS ? sb - q0 : kFavMaxS;
+    hipLaunchKernelGGL(k_fav, grid, dim3(256), 0, st, e.YT, e.XtT, e.W, e.V, e.Np, e.npd, q0, cnt);
+  }
+}
+
+void launch_fav_rot_acc(const Ensemble &e, int s0, int sb, hipStream_t st) {
+  hipLaunchKernelGGL(k_fav_rot_acc, dim3((e.n + 255) / 256), dim3(256), 0, st, e.V, e.f_acc, e.S,
+                     e.meta, e.n, e.npd, e.sym_stride, s0, sb);
+}
+
+void launch_zero(double *p, size_t count, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, grid1(count, 256), dim3(256), 0, st, p, count);
+}
+
+void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
+                     hipStream_t st) {
+  double inv = 1.0 / (e.N_eff * (double)e.N_sym);
+  int do_f = (what & kTermF) && f_out;
+  int do_d = (what & (kTermD2vR | kTermD2vY)) && d2v_out;
+  dim3 grid((e.n + 255) / 256, e.n);
+  hipLaunchKernelGGL(k_finalize, grid, dim3(256), 0, st, e.f_acc, e.d2v_acc, f_out, d2v_out, e.n,
+                     e.npd, inv, do_f, do_d, (int)d2v_accumulate);
+}
+
+void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st) {
+  size_t count = (size_t)sb * e.npd * e.npd;
+  hipLaunchKernelGGL(k_split_ys, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, count, prec);
+}
+
+}  // namespace tdscha

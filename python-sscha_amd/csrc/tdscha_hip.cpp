@@ -1,0 +1,517 @@
+// C ABI of libtdscha_hip: ensemble upload, per-pass orchestration of the kernels,
+// and the drop-in host-pointer entry points.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "tdscha_internal.h"
+#include "../../include/tdscha_hip.h"
+
+namespace tdscha {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+}  // namespace tdscha
+
+using namespace tdscha;
+
+struct tdscha_ensemble {
+  Ensemble e;
+  std::vector<void *> allocs;
+  long long bytes = 0;
+};
+
+#define TD_CHECK(expr)                                                                    \
+  do {                                                                                    \
+    hipError_t _err = (expr);                                                             \
+    if (_err != hipSuccess) {                                                             \
+      set_error(std::string("HIP error at ") + __FILE__ + ":" + std::to_string(__LINE__) + \
+                ": " + hipGetErrorString(_err) + " (" #expr ")");                         \
+      return 10;                                                                          \
+    }                                                                                     \
+  } while (0)
+
+namespace {
+
+template <class T>
+int dev_alloc(tdscha_ensemble *h, T **p, size_t count) {
+  void *ptr = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = sizeof(T);
+  hipError_t err = hipMalloc(&ptr, bytes);
+  if (err != hipSuccess) {
+    set_error(std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
+              hipGetErrorString(err));
+    return 11;
+  }
+  h->allocs.push_back(ptr);
+  h->bytes += (long long)bytes;
+  *p = static_cast<T *>(ptr);
+  return 0;
+}
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+double bose(double w, double T) { return T > 0 ? 1.0 / (std::exp(w * kRyToK / T) - 1.0) : 0.0; }
+
+// Validate the block description (contiguous, sorted, covering every mode once) and
+// fill the per-block offsets.
+int check_blocks(int n, int n_blocks, const int *N_deg, const int *const *deg_space,
+                 const int *blocks_ids, const double *w, Ensemble &e) {
+  if (n_blocks <= 0) { set_error("n_blocks must be positive"); return 2; }
+  std::vector<int> seen(n, 0);
+  e.blk_off.assign(n_blocks, 0);
+  e.blk_dim.assign(n_blocks, 0);
+  e.sym_off.assign(n_blocks, 0);
+  int stride = 0, maxd = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    int d = N_deg[b];
+    if (d <= 0) { set_error("block " + std::to_string(b) + " has non-positive size"); return 2; }
+    int m0 = deg_space[b][0];
+    for (int k = 0; k < d; ++k) {
+      int m = deg_space[b][k];
+      if (m != m0 + k || m < 0 || m >= n) {
+        set_error("degenerate_space[" + std::to_string(b) +
+                  "] must list contiguous sorted modes inside [0, n_modes)");
+        return 2;
+      }
+      if (seen[m]++) { set_error("mode " + std::to_string(m) + " belongs to two blocks"); return 2; }
+      if (blocks_ids[m] != b) {
+        set_error("blocks_ids[" + std::to_string(m) + "] disagrees with degenerate_space");
+        return 2;
+      }
+      double rel = std::fabs(w[m] - w[m0]) / std::fabs(w[m0]);
+      if (!(rel <= 1e-8)) {
+        set_error("frequencies inside symmetry block " + std::to_string(b) +
+                  " differ by more than 1e-8 relative: blocks must hold degenerate modes");
+        return 3;
+      }
+    }
+    e.blk_off[b] = m0;
+    e.blk_dim[b] = d;
+    e.sym_off[b] = stride;
+    stride += d * d;
+    if (d > maxd) maxd = d;
+  }
+  for (int m = 0; m < n; ++m)
+    if (!seen[m]) { set_error("mode " + std::to_string(m) + " is in no block"); return 2; }
+  e.sym_stride = stride;
+  e.max_block = maxd;
+  return 0;
+}
+
+int choose_s_batch(const Ensemble &e, int requested) {
+  if (requested > 0) return requested < e.N_sym ? requested : e.N_sym;
+  const long per1 = e.Np / kPadRows;
+  const long per2 = (long)(e.npd / kPadCols) * (e.npd / kPadCols);
+  long per = per1 < per2 ? per1 : per2;
+  long sb = (4096 + per - 1) / per;
+  // workspace per symmetry: Tw, Ys, B (npd^2 doubles each) + Ys_lp + W + G + V
+  const double per_s_bytes = 3.0 * 8.0 * e.npd * (double)e.npd + 4.0 * e.npd * (double)e.npd +
+                             8.0 * (4.0 * e.Np + 3.0 * e.npd);
+  const double budget = 6.0e9;
+  long cap = (long)(budget / per_s_bytes);
+  if (cap < 1) cap = 1;
+  if (sb > cap) sb = cap;
+  if (sb > e.N_sym) sb = e.N_sym;
+  if (sb < 1) sb = 1;
+  return (int)sb;
+}
+
+// Per-pass orchestration of one ensemble application. Y1/R1 must already be staged in
+// e.Ysym / e.R1p (prep or unpack kernel). Leaves unnormalised sums in f_acc / d2v_acc.
+int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
+  const bool need_q = what & (kTermF | kTermD2vY);
+  const bool need_r = what & kTermD2vR;
+  const bool need_d = what & (kTermD2vR | kTermD2vY);
+  const bool need_f = what & kTermF;
+  launch_zero(e.f_acc, e.npd, st);
+  launch_zero(e.d2v_acc, (size_t)e.npd * e.npd, st);
+  if (!need_r) launch_zero(e.G, (size_t)e.s_batch * 2 * e.npd, st);
+  for (int s0 = 0; s0 < e.N_sym; s0 += e.s_batch) {
+    const int sb = (e.N_sym - s0) < e.s_batch ? (e.N_sym - s0) : e.s_batch;
+    if (need_q) {
+      launch_rot_right(e, e.Ysym, 0, false, e.Tw, s0, sb, false, st);
+      launch_rot_left(e, e.Tw, e.Ys, s0, sb, true, false, st);
+      if (prec != kFP64) launch_split_ys(e, sb, prec, st);
+    }
+    if (need_r) launch_rot_vec(e, s0, sb, st);
+    launch_gemm1(e, sb, prec, what, st);
+    if (need_d) {
+      launch_gemm2(e, sb, prec, st);
+      launch_rot_right(e, e.B, (long)e.npd * e.npd, true, e.Tw, s0, sb, true, st);
+      launch_rot_left(e, e.Tw, e.d2v_acc, s0, sb, false, true, st);
+    }
+    if (need_f) {
+      launch_fav(e, sb, st);
+      launch_fav_rot_acc(e, s0, sb, st);
+    }
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_error(std::string("kernel launch failed: ") + hipGetErrorString(err));
+    return 12;
+  }
+  return 0;
+}
+
+int check_prec(const Ensemble &e, int prec) {
+  if (prec != kFP64 && prec != kFP32 && prec != kBF16x3) {
+    set_error("unknown precision " + std::to_string(prec));
+    return 4;
+  }
+  if (prec != kFP64 && !(e.prec_mask & (1 << prec))) {
+    set_error("ensemble was created without the operand copies of precision " + std::to_string(prec));
+    return 4;
+  }
+  if (prec != kFP64) {
+    set_error("precision " + std::to_string(prec) + " is not available in this build");
+    return 4;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tdscha_last_error(void) { return g_err.c_str(); }
+int tdscha_abi_version(void) { return TDSCHA_ABI_VERSION; }
+
+int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, const double *Y,
+                           const double *w, const double *rho, double T, int n_modes, int n_configs,
+                           int N_sym, int n_blocks, const double *const *symmetries,
+                           const int *N_degeneracy, const int *const *degenerate_space,
+                           const int *blocks_ids, int precisions, int s_batch) {
+  if (!out) { set_error("out is NULL"); return 1; }
+  *out = nullptr;
+  if (n_modes <= 0 || n_configs <= 0 || N_sym <= 0) {
+    set_error("n_modes, n_configs and N_sym must be positive");
+    return 1;
+  }
+  if (!X || !Y || !w || !rho || !symmetries || !N_degeneracy || !degenerate_space || !blocks_ids) {
+    set_error("NULL input pointer");
+    return 1;
+  }
+  for (int m = 0; m < n_modes; ++m)
+    if (!(w[m] > 0) || !std::isfinite(w[m])) {
+      set_error("frequency " + std::to_string(m) + " is not positive and finite");
+      return 3;
+    }
+  tdscha_ensemble *h = new (std::nothrow) tdscha_ensemble();
+  if (!h) { set_error("out of host memory"); return 11; }
+  Ensemble &e = h->e;
+  int rc = check_blocks(n_modes, n_blocks, N_degeneracy, degenerate_space, blocks_ids, w, e);
+  if (rc) { delete h; return rc; }
+  TD_CHECK(hipSetDevice(device));
+  e.device = device;
+  e.N = n_configs;
+  e.n = n_modes;
+  e.Np = round_up(n_configs, kPadRows);
+  e.npd = round_up(n_modes, kPadCols);
+  e.N_sym = N_sym;
+  e.n_blocks = n_blocks;
+  e.T = T;
+  e.prec_mask = (precisions | 1) & 7;
+  double neff = 0.0;
+  for (int i = 0; i < n_configs; ++i) neff += rho[i];
+  e.N_eff = neff;
+  e.s_batch = choose_s_batch(e, s_batch);
+
+  const int N = e.N, n = e.n, Np = e.Np, npd = e.npd;
+  std::vector<double> fu(npd, 0.0), fp(npd, 0.0), nocc(n, 0.0);
+  for (int m = 0; m < n; ++m) {
+    double nw = bose(w[m], T);
+    nocc[m] = nw;
+    fu[m] = 2 * w[m] / (1 + 2 * nw);   // LanczosFunctions.c:14-21
+    fp[m] = (1 + 2 * nw) / (2 * w[m]);  // LanczosFunctions.c:24-30
+  }
+  std::vector<double> hX((size_t)Np * npd, 0.0), hFt((size_t)Np * npd, 0.0);
+  std::vector<double> hXtT((size_t)npd * Np, 0.0), hYT((size_t)npd * Np, 0.0), hrho(Np, 0.0);
+  for (int i = 0; i < N; ++i) {
+    hrho[i] = rho[i];
+    for (int m = 0; m < n; ++m) {
+      double x = X[(size_t)i * n + m], y = Y[(size_t)i * n + m];
+      hX[(size_t)i * npd + m] = x;
+      hFt[(size_t)i * npd + m] = fp[m] * y;
+      hXtT[(size_t)m * Np + i] = fu[m] * x;
+      hYT[(size_t)m * Np + i] = y;
+    }
+  }
+  std::vector<ModeMeta> meta(npd);
+  for (int m = 0; m < npd; ++m) meta[m] = ModeMeta{0, 1, 0, 0};
+  for (int m = 0; m < n; ++m) {
+    int b = blocks_ids[m];
+    meta[m] = ModeMeta{e.blk_off[b], e.blk_dim[b], e.sym_off[b], 0};
+  }
+
+  rc = 0;
+  rc |= dev_alloc(h, &e.X, hX.size());
+  rc |= dev_alloc(h, &e.Ft, hFt.size());
+  rc |= dev_alloc(h, &e.XtT, hXtT.size());
+  rc |= dev_alloc(h, &e.YT, hYT.size());
+  rc |= dev_alloc(h, &e.rho, (size_t)Np);
+  rc |= dev_alloc(h, &e.fups, (size_t)npd);
+  rc |= dev_alloc(h, &e.fpsi, (size_t)npd);
+  rc |= dev_alloc(h, &e.w, (size_t)n);
+  rc |= dev_alloc(h, &e.nocc, (size_t)n);
+  rc |= dev_alloc(h, &e.S, (size_t)N_sym * e.sym_stride);
+  rc |= dev_alloc(h, &e.meta, (size_t)npd);
+  const size_t sb = e.s_batch, nn = (size_t)npd * npd;
+  rc |= dev_alloc(h, &e.Ysym, nn);
+  rc |= dev_alloc(h, &e.R1p, (size_t)npd);
+  rc |= dev_alloc(h, &e.Tw, sb * nn);
+  rc |= dev_alloc(h, &e.Ys, sb * nn);
+  rc |= dev_alloc(h, &e.G, sb * 2 * npd);
+  rc |= dev_alloc(h, &e.W, sb * 4 * Np);
+  rc |= dev_alloc(h, &e.B, sb * nn);
+  rc |= dev_alloc(h, &e.V, sb * npd);
+  rc |= dev_alloc(h, &e.d2v_acc, nn);
+  rc |= dev_alloc(h, &e.f_acc, (size_t)npd);
+  if (rc) { tdscha_ensemble_destroy(h); return 11; }
+
+  auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+    hipError_t err = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+      set_error(std::string("upload failed: ") + hipGetErrorString(err));
+      return 10;
+    }
+    return 0;
+  };
+  rc |= up(e.X, hX.data(), hX.size() * 8);
+  rc |= up(e.Ft, hFt.data(), hFt.size() * 8);
+  rc |= up(e.XtT, hXtT.data(), hXtT.size() * 8);
+  rc |= up(e.YT, hYT.data(), hYT.size() * 8);
+  rc |= up(e.rho, hrho.data(), hrho.size() * 8);
+  rc |= up(e.fups, fu.data(), fu.size() * 8);
+  rc |= up(e.fpsi, fp.data(), fp.size() * 8);
+  rc |= up(e.w, w, (size_t)n * 8);
+  rc |= up(e.nocc, nocc.data(), nocc.size() * 8);
+  rc |= up(e.meta, meta.data(), meta.size() * sizeof(ModeMeta));
+  for (int b = 0; b < n_blocks && !rc; ++b) {
+    const size_t d2 = (size_t)e.blk_dim[b] * e.blk_dim[b];
+    hipError_t err = hipMemcpy2D(e.S + e.sym_off[b], (size_t)e.sym_stride * 8, symmetries[b], d2 * 8,
+                                 d2 * 8, (size_t)N_sym, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+      set_error(std::string("symmetry upload failed: ") + hipGetErrorString(err));
+      rc = 10;
+    }
+  }
+  if (rc) { tdscha_ensemble_destroy(h); return rc; }
+  *out = h;
+  return 0;
+}
+
+int tdscha_ensemble_destroy(tdscha_ensemble *h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->e.device);
+  for (void *p : h->allocs) (void)hipFree(p);
+  delete h;
+  return 0;
+}
+
+int tdscha_ensemble_dims(const tdscha_ensemble *h, int *d) {
+  if (!h || !d) { set_error("NULL argument"); return 1; }
+  d[0] = h->e.n; d[1] = h->e.N; d[2] = h->e.N_sym; d[3] = h->e.s_batch;
+  d[4] = h->e.npd; d[5] = h->e.Np;
+  return 0;
+}
+
+long long tdscha_ensemble_bytes(const tdscha_ensemble *h) { return h ? h->bytes : 0; }
+
+int tdscha_ensemble_apply(tdscha_ensemble *h, int precision, int terms, const double *R1,
+                          const double *Y1, double *f, double *d2v, int d2v_accumulate,
+                          void *stream) {
+  if (!h) { set_error("NULL ensemble"); return 1; }
+  Ensemble &e = h->e;
+  int rc = check_prec(e, precision);
+  if (rc) return rc;
+  if (terms <= 0 || terms > 7) { set_error("terms must be a non-empty mask of TDSCHA_TERM_*"); return 1; }
+  if ((terms & (kTermF | kTermD2vY)) && !Y1) { set_error("Y1 is NULL"); return 1; }
+  if ((terms & kTermD2vR) && !R1) { set_error("R1 is NULL"); return 1; }
+  if ((terms & kTermF) && !f) { set_error("f is NULL"); return 1; }
+  if ((terms & (kTermD2vR | kTermD2vY)) && !d2v) { set_error("d2v is NULL"); return 1; }
+  TD_CHECK(hipSetDevice(e.device));
+  hipStream_t st = (hipStream_t)stream;
+  // Stage inputs; a NULL operand is staged as zeros.
+  launch_prep_inputs(e, R1, Y1, st);
+  rc = run_passes(e, precision, terms, st);
+  if (rc) return rc;
+  launch_finalize(e, terms, d2v_accumulate != 0, f, d2v, st);
+  TD_CHECK(hipGetLastError());
+  return 0;
+}
+
+int tdscha_apply_full_L(tdscha_ensemble *h, int precision, int flags, double shift,
+                        const double *psi, double *out, void *stream) {
+  if (!h || !psi || !out) { set_error("NULL argument"); return 1; }
+  if (psi == out) { set_error("out must not alias psi"); return 1; }
+  Ensemble &e = h->e;
+  int rc = check_prec(e, precision);
+  if (rc) return rc;
+  TD_CHECK(hipSetDevice(e.device));
+  hipStream_t st = (hipStream_t)stream;
+  if (!(flags & kOpIgnoreV3)) {
+    launch_unpack(e, psi, flags, st);
+    int what = kTermF | kTermD2vR | ((flags & kOpIgnoreV4) ? 0 : kTermD2vY);
+    rc = run_passes(e, precision, what, st);
+    if (rc) return rc;
+  }
+  launch_post(e, psi, out, flags, shift, st);
+  TD_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---- host-pointer drop-ins --------------------------------------------------------------
+
+static int host_call(const double *X, const double *Y, const double *w, const double *rho,
+                     const double *R1, const double *Y1, double T, int terms, int n, int N,
+                     int N_sym, int n_blocks, const double *const *syms, const int *ndeg,
+                     const int *const *degs, const int *bids, double *f_out, double *d2v_out,
+                     int d2v_accumulate) {
+  if (!X || !Y || !w || !rho || (!f_out && (terms & kTermF)) ||
+      (!d2v_out && (terms & (kTermD2vR | kTermD2vY)))) {
+    set_error("NULL argument");
+    return 1;
+  }
+  int dev = 0;
+  TD_CHECK(hipGetDevice(&dev));
+  tdscha_ensemble *h = nullptr;
+  int rc = tdscha_ensemble_create(&h, dev, X, Y, w, rho, T, n, N, N_sym, n_blocks, syms, ndeg, degs,
+                                  bids, 1, 0);
+  if (rc) return rc;
+  double *dR1 = nullptr, *dY1 = nullptr, *df = nullptr, *dd = nullptr;
+  auto cleanup = [&]() {
+    if (dR1) (void)hipFree(dR1);
+    if (dY1) (void)hipFree(dY1);
+    if (df) (void)hipFree(df);
+    if (dd) (void)hipFree(dd);
+    tdscha_ensemble_destroy(h);
+  };
+  const size_t nn = (size_t)n * n;
+  if (hipMalloc(&dR1, n * 8) != hipSuccess || hipMalloc(&dY1, nn * 8) != hipSuccess ||
+      hipMalloc(&df, n * 8) != hipSuccess || hipMalloc(&dd, nn * 8) != hipSuccess) {
+    set_error("hipMalloc failed for the call buffers");
+    cleanup();
+    return 11;
+  }
+  hipError_t err = hipSuccess;
+  err = R1 ? hipMemcpy(dR1, R1, n * 8, hipMemcpyHostToDevice) : hipMemset(dR1, 0, n * 8);
+  if (err == hipSuccess)
+    err = Y1 ? hipMemcpy(dY1, Y1, nn * 8, hipMemcpyHostToDevice) : hipMemset(dY1, 0, nn * 8);
+  if (err == hipSuccess && d2v_accumulate) err = hipMemcpy(dd, d2v_out, nn * 8, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    set_error(std::string("input copy failed: ") + hipGetErrorString(err));
+    cleanup();
+    return 10;
+  }
+  rc = tdscha_ensemble_apply(h, kFP64, terms, dR1, dY1, df, dd, d2v_accumulate, nullptr);
+  if (!rc) {
+    err = hipDeviceSynchronize();
+    if (err == hipSuccess && (terms & kTermF)) err = hipMemcpy(f_out, df, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess && (terms & (kTermD2vR | kTermD2vY)))
+      err = hipMemcpy(d2v_out, dd, nn * 8, hipMemcpyDeviceToHost);
+    if (err != hipSuccess) {
+      set_error(std::string("device execution failed: ") + hipGetErrorString(err));
+      rc = 10;
+    }
+  }
+  cleanup();
+  return rc;
+}
+
+int tdscha_get_f_average_from_Y_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                              const double *Y1, double T, int n_modes, int n_configs,
+                                              const double *w_is, const double *const *symmetries,
+                                              int N_sym, const int *N_degeneracy,
+                                              const int *const *degenerate_space,
+                                              const int *blocks_ids, double *f_average) {
+  // n_blocks is implicit in the reference's arguments: one past the largest block id.
+  int nb = 0;
+  for (int m = 0; m < n_modes; ++m) nb = blocks_ids[m] + 1 > nb ? blocks_ids[m] + 1 : nb;
+  return host_call(X, Y, w, w_is, nullptr, Y1, T, kTermF, n_modes, n_configs, N_sym, nb, symmetries,
+                   N_degeneracy, degenerate_space, blocks_ids, f_average, nullptr, 0);
+}
+
+int tdscha_get_d2v_dR2_from_R_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                            const double *R1, double T, int n_modes, int n_configs,
+                                            const double *w_is, const double *const *symmetries,
+                                            int N_sym, const int *N_degeneracy,
+                                            const int *const *degenerate_space,
+                                            const int *blocks_ids, double *d2v_dR2) {
+  int nb = 0;
+  for (int m = 0; m < n_modes; ++m) nb = blocks_ids[m] + 1 > nb ? blocks_ids[m] + 1 : nb;
+  return host_call(X, Y, w, w_is, R1, nullptr, T, kTermD2vR, n_modes, n_configs, N_sym, nb,
+                   symmetries, N_degeneracy, degenerate_space, blocks_ids, nullptr, d2v_dR2, 0);
+}
+
+int tdscha_get_d2v_dR2_from_Y_pert_sym_fast(const double *X, const double *Y, const double *w,
+                                            const double *Y1, double T, int n_modes, int n_configs,
+                                            const double *w_is, const double *const *symmetries,
+                                            int N_sym, const int *N_degeneracy,
+                                            const int *const *degenerate_space,
+                                            const int *blocks_ids, double *d2v_dR2_out) {
+  int nb = 0;
+  for (int m = 0; m < n_modes; ++m) nb = blocks_ids[m] + 1 > nb ? blocks_ids[m] + 1 : nb;
+  return host_call(X, Y, w, w_is, nullptr, Y1, T, kTermD2vY, n_modes, n_configs, N_sym, nb,
+                   symmetries, N_degeneracy, degenerate_space, blocks_ids, nullptr, d2v_dR2_out, 1);
+}
+
+int tdscha_get_perturb_average_sym(const double *X, const double *Y, const double *w,
+                                   const double *rho, const double *R1, const double *Y1, double T,
+                                   int apply_D4, int N_syms, int n_blocks,
+                                   const double *const *sym_list, const int *n_deg,
+                                   const int *const *sym_basis_list, const int *blocks_ids,
+                                   int N_configs, int N_modes, double *force_out, double *d2v_out) {
+  if (apply_D4 != 0 && apply_D4 != 1) {
+    set_error("the apply_D4 value passed must be either 0 or 1");
+    return 2;
+  }
+  int terms = kTermF | kTermD2vR | (apply_D4 ? kTermD2vY : 0);
+  return host_call(X, Y, w, rho, R1, Y1, T, terms, N_modes, N_configs, N_syms, n_blocks, sym_list,
+                   n_deg, sym_basis_list, blocks_ids, force_out, d2v_out, 0);
+}
+
+static void die_on(int rc, const char *fn) {
+  if (rc) {
+    fprintf(stderr, "Error in %s: %s\n", fn, tdscha_last_error());
+    exit(EXIT_FAILURE);
+  }
+}
+
+void get_f_average_from_Y_pert_sym_fast(double *X, double *Y, double *w, double *Y1, double T,
+                                        int n_modes, int n_configs, double *w_is, double **symmetries,
+                                        int N_sym, int *N_degeneracy, int **degenerate_space,
+                                        int *blocks_ids, double *f_average) {
+  die_on(tdscha_get_f_average_from_Y_pert_sym_fast(X, Y, w, Y1, T, n_modes, n_configs, w_is,
+                                                   symmetries, N_sym, N_degeneracy,
+                                                   degenerate_space, blocks_ids, f_average),
+         __func__);
+}
+
+void get_d2v_dR2_from_R_pert_sym_fast(double *X, double *Y, double *w, double *R1, double T,
+                                      int n_modes, int n_configs, double *w_is, double **symmetries,
+                                      int N_sym, int *N_degeneracy, int **degenerate_space,
+                                      int *blocks_ids, double *d2v_dR2) {
+  die_on(tdscha_get_d2v_dR2_from_R_pert_sym_fast(X, Y, w, R1, T, n_modes, n_configs, w_is,
+                                                 symmetries, N_sym, N_degeneracy, degenerate_space,
+                                                 blocks_ids, d2v_dR2),
+         __func__);
+}
+
+void get_d2v_dR2_from_Y_pert_sym_fast(double *X, double *Y, double *w, double *Y1, double T,
+                                      int n_modes, int n_configs, double *w_is, double **symmetries,
+                                      int N_sym, int *N_degeneracy, int **degenerate_space,
+                                      int *blocks_ids, double *d2v_dR2_out) {
+  die_on(tdscha_get_d2v_dR2_from_Y_pert_sym_fast(X, Y, w, Y1, T, n_modes, n_configs, w_is,
+                                                 symmetries, N_sym, N_degeneracy, degenerate_space,
+                                                 blocks_ids, d2v_dR2_out),
+         __func__);
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP translation units of libtdscha_hip.
+//
+// Device data layout of one ensemble (all row-major, zero padded):
+//   Np  = N rounded up to a multiple of kPadRows (configs)
+//   npd = n rounded up to a multiple of kPadCols (modes)
+//   X   [Np x npd]  fp64  displacements in the mode basis          (GEMM1 A operand)
+//   Ft  [Np x npd]  fp64  f_psi * Y (scaled forces)                 (GEMM1 epilogue)
+//   XtT [npd x Np]  fp64  (f_ups * X)^T                              (GEMM2 A operand)
+//   YT  [npd x Np]  fp64  Y^T                                        (GEMM2 B operand source)
+//   rho [Np]        fp64  importance weights (0 in the padding)
+//   S   [N_sym x sym_stride] fp64  block-diagonal symmetry matrices, per symmetry the
+//        blocks packed one after the other, each d_b x d_b row-major.
+//   Reduced-precision copies of the GEMM operands (bf16 hi/lo pairs, fp32) live
+//   next to them when the ensemble was created with those precisions enabled.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tdscha {
+
+constexpr int kPadRows = 64;   // config padding (GEMM1 row tile, GEMM2 k tile)
+constexpr int kPadCols = 64;   // mode padding (GEMM tiles along modes)
+constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
+
+enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
+enum Terms { kTermF = 1, kTermD2vR = 2, kTermD2vY = 4 };
+enum OpFlags {
+  kOpTranspose = 1, kOpWigner = 2, kOpIgnoreV3 = 4, kOpIgnoreV4 = 8, kOpIgnoreHarmonic = 16,
+  kOpReverse = 32
+};
+
+// Per-mode symmetry metadata on the device.
+struct ModeMeta {
+  int blk_off;   // first mode of the block this mode belongs to
+  int blk_dim;   // block size d_b
+  int sym_off;   // offset of the block inside one symmetry's packed matrices
+  int pad;
+};
+
+struct Ensemble {
+  int device = 0;
+  int N = 0, n = 0, Np = 0, npd = 0;
+  int N_sym = 0, n_blocks = 0, sym_stride = 0, max_block = 0;
+  int s_batch = 0;            // symmetries processed per pass
+  int prec_mask = 0;          // precisions whose operand copies exist
+  double T = 0.0, N_eff = 0.0;
+  std::vector<int> blk_off, blk_dim, sym_off;
+
+  // fp64 operands
+  double *X = nullptr, *Ft = nullptr, *XtT = nullptr, *YT = nullptr, *rho = nullptr;
+  double *fups = nullptr, *fpsi = nullptr, *S = nullptr;
+  double *w = nullptr, *nocc = nullptr;   // frequencies [n] and Bose occupations [n]
+  ModeMeta *meta = nullptr;
+  // reduced-precision operand copies
+  __hip_bfloat16 *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
+  float *X_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
+
+  // workspace (per pass of s_batch symmetries)
+  double *Ysym = nullptr;     // [npd x npd]  (Y1 + Y1^T)/2, padded
+  double *R1p = nullptr;      // [npd]
+  double *Tw = nullptr;       // [s_batch x npd x npd]  rotation intermediate
+  double *Ys = nullptr;       // [s_batch x npd x npd]  S_s^T Ysym S_s
+  void *Ys_lp = nullptr;      // reduced-precision Ys (bf16 hi+lo or fp32)
+  double *G = nullptr;        // [s_batch x 2 x npd]  g1 = f_ups*S^T R1, g2 = S^T R1 / f_psi
+  double *W = nullptr;        // [s_batch x 4 x Np]  c1, c2, e1, e2 per image
+  double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
+  double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
+  double *d2v_acc = nullptr;  // [npd x npd]
+  double *f_acc = nullptr;    // [npd]
+};
+
+void set_error(const std::string &msg);
+
+// ---- kernels / launchers (rotate.hip) ----
+void launch_prep_inputs(const Ensemble &e, const double *R1, const double *Y1, hipStream_t st);
+void launch_rot_right(const Ensemble &e, const double *in, long in_stride_s, bool symmetrize,
+                      double *out, int s0, int sb, bool transS, hipStream_t st);
+void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, int sb, bool transS,
+                     bool accumulate, hipStream_t st);
+void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st);
+void launch_fav(const Ensemble &e, int sb, hipStream_t st);
+void launch_fav_rot_acc(const Ensemble &e, int s0, int sb, hipStream_t st);
+void launch_zero(double *p, size_t count, hipStream_t st);
+void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
+                     hipStream_t st);
+void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st);
+
+// ---- operator pre/post transforms and harmonic part (operator.hip) ----
+void launch_unpack(const Ensemble &e, const double *psi, int flags, hipStream_t st);
+void launch_post(const Ensemble &e, const double *psi, double *out, int flags, double shift,
+                 hipStream_t st);
+
+// ---- GEMMs (gemm.hip) ----
+// GEMM1: per image (config i, symmetry s) the quadratic forms
+//   alpha = x_i^T Ys_s x_i, beta = ft_i^T Ys_s x_i, gamma = x_i . g1_s, delta = ft_i . g2_s
+// and the per-image weights (c1, c2, e1, e2) written to W.
+void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st);
+// GEMM2: B_s = X~^T (c1_s * Y + c2_s/2 * X~) for each symmetry in the pass.
+void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st);
+
+}  // namespace tdscha

@@ -1,0 +1,584 @@
+"""Lanczos with the ensemble L-application on an MI355X.
+
+Mirrors the interface of the reference class ``tdscha.DynamicalLanczos.Lanczos``
+(Modules/DynamicalLanczos.py, "Py" below) for the path that every Lanczos step takes:
+apply_full_L -> apply_anharmonic_FT -> GetPerturbAverageSym. Three new modes select the
+device path (next to MODE_FAST_* at Py:166-169):
+
+    MODE_GPU_FP64    plain FP64 contractions, reproduces the reference to <= 1e-10
+    MODE_GPU_FP32    FP32 MFMA contractions, FP64 everything else
+    MODE_GPU_BF16X3  BF16 x3 split contractions, FP64 everything else
+
+Everything on the Krylov side stays FP64 on the device: the harmonic part L1, the
+psi <-> (R1, Y1) transforms, the recurrence scalars a, b, c, the optional Gram-Schmidt
+and the continued fraction.
+
+Deviations from the reference, on purpose:
+  * The ensemble is given already projected on the modes (X, Y, w, rho, T and the
+    symmetry blocks), as Lanczos.__init__ leaves it (Py:380-442) and save_status stores it
+    (Py:3617-3646); CellConstructor / sscha objects are not required.
+  * Errors raise ValueError / RuntimeError instead of exiting the interpreter.
+  * run_FT(full_gs=True) adds a two-sided Gram-Schmidt against every stored vector with
+    the biorthogonal duals p_j * s_norm_j / c_(j-1); n_rep_orth keeps the reference's
+    own loop (Py:6245-6317) verbatim.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .ensemble import DeviceEnsemble, SymmetryBlocks
+
+TYPE_DP = np.double
+__EPSILON__ = 1e-12          # Py:121
+RY_TO_K = 157887.32400374097  # Py:134
+
+MODE_SLOW_SERIAL = 0
+MODE_FAST_SERIAL = 1
+MODE_FAST_MPI = 2
+MODE_FAST_JULIA = 3
+MODE_GPU_FP64 = 4
+MODE_GPU_FP32 = 5
+MODE_GPU_BF16X3 = 6
+_GPU_PRECISION = {MODE_GPU_FP64: "fp64", MODE_GPU_FP32: "fp32", MODE_GPU_BF16X3: "bf16x3"}
+
+
+def f_ups(w, T):
+    """Py:144-155."""
+    w = np.asarray(w, dtype=np.float64)
+    n_w = 1 / (np.exp(w * RY_TO_K / T) - 1) if T > 0 else 0
+    return 2 * w / (1 + 2 * n_w)
+
+
+class Lanczos(object):
+    """Biconjugate / symmetric Lanczos on the TD-SCHA L operator (Py:182)."""
+
+    def __init__(self, ensemble=None, mode=MODE_GPU_FP64, use_wigner=False, device=0,
+                 s_batch=0):
+        if mode not in _GPU_PRECISION:
+            raise ValueError("mode %r: this build implements the device modes MODE_GPU_FP64, "
+                             "MODE_GPU_FP32 and MODE_GPU_BF16X3" % (mode,))
+        self.mode = mode
+        self.device = device
+        self.s_batch = s_batch
+        self.verbose = True
+        self.T = 0.0
+        self.nat = 0
+        self.m = np.zeros(0)
+        self.w = np.zeros(0)
+        self.pols = np.zeros((0, 0))
+        self.n_modes = 0
+        self.ignore_harmonic = False
+        self.ignore_v3 = False
+        self.ignore_v4 = False
+        self.N = 0
+        self.rho = np.zeros(0)
+        self.N_eff = 0
+        self.X = np.zeros((0, 0))
+        self.Y = np.zeros((0, 0))
+        self.psi = np.zeros(0)
+        self.a_coeffs = []
+        self.b_coeffs = []
+        self.c_coeffs = []
+        self.basis_P = []
+        self.basis_Q = []
+        self.s_norm = []
+        self.reverse_L = False
+        self.shift_value = 0
+        self.symmetries = None
+        self.degenerate_space = None
+        self.N_degeneracy = None
+        self.sym_block_id = None
+        self.n_syms = 1
+        self.initialized = False
+        self.perturbation_modulus = 1
+        self.use_wigner = use_wigner
+        self.ignore_small_w = False
+        self.L_linop = None
+        self._dev_ens = None
+        if ensemble is not None:
+            self.set_ensemble(ensemble.X, ensemble.Y, ensemble.w, ensemble.rho, ensemble.T,
+                              getattr(ensemble, "pols", None), getattr(ensemble, "m", None))
+            sym = getattr(ensemble, "sym", None)
+            if sym is not None:
+                self.set_symmetries(sym.symmetries, sym.N_degeneracy, sym.degenerate_space,
+                                    sym.block_id)
+
+    # ------------------------------------------------------------------ setup
+    def set_ensemble(self, X, Y, w, rho, T, pols=None, m=None):
+        """The projected ensemble (the arrays Py:380-442 builds)."""
+        self.X = np.ascontiguousarray(X, dtype=TYPE_DP)
+        self.Y = np.ascontiguousarray(Y, dtype=TYPE_DP)
+        self.w = np.ascontiguousarray(w, dtype=TYPE_DP)
+        self.rho = np.ascontiguousarray(rho, dtype=TYPE_DP)
+        self.T = float(T)
+        self.N, self.n_modes = self.X.shape
+        self.N_eff = np.sum(self.rho)
+        if pols is not None:
+            self.pols = np.asarray(pols, dtype=TYPE_DP)
+        if m is not None:
+            self.m = np.asarray(m, dtype=TYPE_DP)
+            self.nat = len(self.m) // 3
+        n = self.n_modes
+        self.psi = np.zeros(n + n * (n + 1), dtype=TYPE_DP)
+        self._dev_ens = None
+        self.initialized = False
+
+    def set_symmetries(self, symmetries, N_degeneracy, degenerate_space, block_id):
+        """Mode-basis symmetry blocks (what prepare_symmetrization stores, Py:741-778)."""
+        self.symmetries = [np.ascontiguousarray(s, dtype=TYPE_DP) for s in symmetries]
+        self.N_degeneracy = np.ascontiguousarray(N_degeneracy, dtype=np.intc)
+        self.degenerate_space = [np.ascontiguousarray(d, dtype=np.intc) for d in degenerate_space]
+        self.sym_block_id = np.ascontiguousarray(block_id, dtype=np.intc)
+        self.n_syms = self.symmetries[0].shape[0]
+        self._dev_ens = None
+        self.initialized = False
+
+    def init(self, use_symmetries=True):
+        """Upload the ensemble to the device (Py:556-575 + prepare_symmetrization)."""
+        if self.symmetries is None or not use_symmetries:
+            if use_symmetries:
+                raise ValueError("no symmetry blocks were given: call set_symmetries() or "
+                                 "init(use_symmetries=False)")
+            sym = SymmetryBlocks.identity(self.n_modes)
+            self.set_symmetries(sym.symmetries, sym.N_degeneracy, sym.degenerate_space,
+                                sym.block_id)
+        sym = SymmetryBlocks(self.symmetries, self.N_degeneracy, self.degenerate_space,
+                             self.sym_block_id)
+        precs = ("fp64", _GPU_PRECISION[self.mode]) if self.mode != MODE_GPU_FP64 else ("fp64",)
+        self._dev_ens = DeviceEnsemble(self.X, self.Y, self.w, self.rho, self.T, sym,
+                                       precisions=precs, device=self.device, s_batch=self.s_batch)
+        self.initialized = True
+        import scipy.sparse.linalg
+        n_psi = len(self.psi)
+        self.L_linop = scipy.sparse.linalg.LinearOperator(
+            shape=(n_psi, n_psi), matvec=self.apply_full_L,
+            rmatvec=lambda p: self.apply_full_L(p, transpose=True), dtype=TYPE_DP)
+
+    def reset(self):
+        """Py:1000-1002 equivalent: forget the Lanczos coefficients and bases."""
+        self.a_coeffs = []
+        self.b_coeffs = []
+        self.c_coeffs = []
+        self.basis_P = []
+        self.basis_Q = []
+        self.s_norm = []
+
+    @property
+    def precision(self):
+        return _GPU_PRECISION[self.mode]
+
+    @property
+    def device_ensemble(self):
+        if not self.initialized:
+            raise ValueError("Error, this class must be initialized before lunching the "
+                             "computation:\n Use the .init()")
+        return self._dev_ens
+
+    # ------------------------------------------------------------------ perturbations
+    def prepare_mode(self, index):
+        """Py:2017-2031."""
+        self.reset()
+        self.psi[:] = 0
+        self.psi[index] = 1
+        self.perturbation_modulus = 1
+
+    def prepare_perturbation(self, vector, masses_exp=1, add=False):
+        """Py:1963-2015 (R sector only)."""
+        if not add:
+            self.reset()
+            self.psi = np.zeros(self.psi.shape, dtype=TYPE_DP)
+        m_on = np.sqrt(self.m) ** masses_exp
+        new_v = np.einsum("a, a, ab->b", m_on, vector, self.pols)
+        self.psi[:self.n_modes] += new_v
+        self.perturbation_modulus = new_v.dot(new_v)
+
+    # ------------------------------------------------------------------ psi helpers
+    def get_Y1(self, half_off_diagonal=False):
+        """Py:2827-2858."""
+        n = self.n_modes
+        nw2 = n * (n + 1) // 2
+        Y1 = np.zeros((n, n))
+        Y1[np.triu_indices(n)] = self.psi[n:n + nw2]
+        Y1 = Y1 + np.triu(Y1, 1).T
+        if half_off_diagonal:
+            Y1 *= 0.5
+            Y1[np.diag_indices(n)] *= 2
+        return Y1
+
+    def get_ReA1(self, half_off_diagonal=False):
+        """Py:2861-2888."""
+        n = self.n_modes
+        nw2 = n * (n + 1) // 2
+        R = np.zeros((n, n))
+        R[np.triu_indices(n)] = self.psi[n + nw2:]
+        R = R + np.triu(R, 1).T
+        if half_off_diagonal:
+            R *= 0.5
+            R[np.diag_indices(n)] *= 2
+        return R
+
+    def mask_dot_wigner(self, debug=False):
+        """Py:5790-5839: weight 2 on off-diagonal pair entries."""
+        n = self.n_modes
+        mask = np.ones(n + n * (n + 1))
+        iu = np.triu_indices(n)
+        off = iu[0] < iu[1]
+        nw2 = n * (n + 1) // 2
+        mask[n:n + nw2][off] = 2
+        mask[n + nw2:][off] = 2
+        return mask
+
+    # ------------------------------------------------------------------ L application
+    def _flags(self, transpose, extra=0):
+        fl = extra
+        if transpose:
+            fl |= L.OP_TRANSPOSE
+        if self.use_wigner:
+            fl |= L.OP_WIGNER
+        if self.ignore_v3:
+            fl |= L.OP_IGNORE_V3
+        if self.ignore_v4:
+            fl |= L.OP_IGNORE_V4
+        if self.ignore_harmonic:
+            fl |= L.OP_IGNORE_HARMONIC
+        if self.reverse_L:
+            fl |= L.OP_REVERSE
+        return fl
+
+    def apply_full_L_device(self, psi, transpose=False, out=None):
+        """L psi with psi already a device tensor; returns a device tensor."""
+        ens = self.device_ensemble
+        return ens.apply_full_L(psi, flags=self._flags(transpose), shift=float(self.shift_value),
+                                precision=self.precision, out=out)
+
+    def apply_full_L(self, target=None, force_t_0=False, force_FT=True, transpose=False,
+                     fast_lanczos=True):
+        """Py:3467-3570 (finite-temperature branch): overwrites and returns self.psi."""
+        if force_t_0 and force_FT:
+            raise ValueError("Error, only one between force_t_0 and force_FT can be True")
+        if target is not None:
+            self.psi = np.asarray(target, dtype=TYPE_DP)
+        t1 = time.time()
+        out = self.apply_full_L_device(self.psi, transpose)
+        self.psi = out.cpu().numpy()
+        if self.verbose:
+            print("Time to apply the full L: {}".format(time.time() - t1))
+        return self.psi
+
+    def apply_L1_FT(self, transpose=False):
+        """Harmonic part only (Py:2227-2385), evaluated on the device."""
+        ens = self.device_ensemble
+        fl = self._flags(transpose) & ~(L.OP_REVERSE | L.OP_IGNORE_HARMONIC)
+        if self.ignore_harmonic:
+            return np.zeros_like(self.psi)
+        return ens.apply_full_L(self.psi, flags=fl | L.OP_IGNORE_V3, shift=0.0,
+                                precision=self.precision).cpu().numpy()
+
+    def apply_anharmonic_FT(self, transpose=False, test_weights=False, use_old_version=False):
+        """Anharmonic part only (Py:2891-3281), evaluated on the device."""
+        ens = self.device_ensemble
+        fl = (self._flags(transpose) & ~(L.OP_REVERSE | L.OP_IGNORE_V3)) | L.OP_IGNORE_HARMONIC
+        return ens.apply_full_L(self.psi, flags=fl, shift=0.0,
+                                precision=self.precision).cpu().numpy()
+
+    # ------------------------------------------------------------------ Lanczos
+    def run_FT(self, n_iter, save_dir=None, save_each=5, verbose=True, n_rep_orth=0, n_ortho=10,
+               flush_output=True, debug=False, prefix="LANCZOS", run_simm=False, optimized=False,
+               full_gs=False):
+        """Biconjugate (or Wigner symmetric) Lanczos, Py:5843-6372, Krylov vectors on device."""
+        import torch
+        self.verbose = verbose
+        ens = self.device_ensemble
+        dev = ens.device
+        psi_host = np.asarray(self.psi, dtype=TYPE_DP)
+        psi_norm = np.sum(psi_host ** 2)
+        if np.isnan(psi_norm) or psi_norm == 0:
+            raise ValueError("Error, you must initialize a perturbation to start the Lanczos.")
+        if run_simm and not self.use_wigner:
+            raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
+        if save_dir is not None and not os.path.exists(save_dir):
+            os.makedirs(save_dir)
+
+        mask = torch.as_tensor(self.mask_dot_wigner(), device=dev) if run_simm else None
+        if run_simm:
+            dot = lambda u, v: torch.dot(u, v * mask)
+        else:
+            dot = lambda u, v: torch.dot(u, v)
+        to_dev = lambda a: torch.as_tensor(np.asarray(a, dtype=TYPE_DP), device=dev).clone()
+
+        i_step = len(self.a_coeffs)
+        if i_step == 0:
+            first = to_dev(psi_host)
+            first = first / torch.sqrt(dot(first, first))
+            Q, P, s_norm_l = [first], [first.clone()], [1.0]
+        else:
+            Q = [to_dev(q) for q in self.basis_Q]
+            P = [to_dev(p) for p in self.basis_P]
+            s_norm_l = [float(s) for s in self.s_norm]
+        a_l = [float(a) for a in self.a_coeffs]
+        b_l = [float(b) for b in self.b_coeffs]
+        c_l = [float(c) for c in self.c_coeffs]
+        if not (len(Q) == len(P) == len(s_norm_l)) or len(b_l) != len(c_l):
+            raise ValueError("Something wrong when restoring the Lanczos.")
+
+        psi_q, psi_p = Q[-1], P[-1]
+        work_q = torch.empty_like(psi_q)
+        work_p = torch.empty_like(psi_q)
+        next_converged = False
+        for i in range(i_step, i_step + n_iter):
+            if verbose:
+                print("\n ===== NEW STEP %d =====\n" % (i + 1))
+                if flush_output:
+                    sys.stdout.flush()
+            L_q = self.apply_full_L_device(psi_q, transpose=False, out=work_q).clone()
+            if run_simm:
+                p_L = L_q.clone()
+            else:
+                p_L = self.apply_full_L_device(psi_p, transpose=True, out=work_p).clone()
+            c_old = c_l[-1] if len(c_l) > 0 else 1.0
+            p_norm = s_norm_l[-1] / c_old
+            a_coeff = float(dot(psi_p, L_q)) * p_norm
+            if np.isnan(a_coeff):
+                raise ValueError("Invalid value encountered during the Lanczos.\n"
+                                 "Check if you have correctly initialized the algorithm.")
+            rk = L_q - a_coeff * psi_q
+            if len(Q) > 1:
+                rk -= c_l[-1] * Q[-2]
+            sk = p_L - a_coeff * psi_p
+            if len(P) > 1:
+                old_p_norm = s_norm_l[-2] if len(c_l) < 2 else s_norm_l[-2] / c_l[-2]
+                sk -= b_l[-1] * P[-2] * (old_p_norm / p_norm)
+            s_norm = float(torch.sqrt(dot(sk, sk)))
+            sk_tilde = sk / s_norm
+            s_norm *= p_norm
+            b_coeff = float(torch.sqrt(dot(rk, rk)))
+            c_coeff = float(dot(sk_tilde, rk / b_coeff)) * s_norm
+            a_l.append(a_coeff)
+            if np.abs(b_coeff) < __EPSILON__ or next_converged:
+                if verbose:
+                    print("Converged (b coefficient is {}, |b| < {})".format(b_coeff, __EPSILON__))
+                break
+            if np.abs(c_coeff) < __EPSILON__:
+                if verbose:
+                    print("Converged (c coefficient is {}, |c| < {})".format(c_coeff, __EPSILON__))
+                break
+            new_q = rk / b_coeff
+            new_p = sk_tilde.clone()
+
+            if full_gs:
+                # two-sided Gram-Schmidt against every stored pair, twice
+                for _rep in range(2):
+                    for j in range(len(P)):
+                        pn = s_norm_l[j] / (c_l[j - 1] if j >= 1 else 1.0)
+                        pd = P[j] * pn
+                        new_q -= dot(pd, new_q) * Q[j]
+                        new_p -= dot(Q[j], new_p) * pd
+                normq = float(torch.sqrt(dot(new_q, new_q)))
+                normp = float(torch.sqrt(dot(new_p, new_p)))
+                if normq < __EPSILON__ or normp < __EPSILON__:
+                    next_converged = True
+                new_q /= normq
+                new_p /= normp
+                s_norm = c_coeff / float(dot(new_p, new_q))
+            for _k in range(n_rep_orth):
+                # the reference's loop, Py:6245-6317, as written
+                ortho_q = 0.0
+                ortho_p = 0.0
+                start = 0 if n_ortho is None else max(0, len(P) - n_ortho)
+                for j in range(start, len(P)):
+                    coeff1 = dot(P[j], new_q)
+                    coeff2 = dot(Q[j], new_p)
+                    new_q -= coeff1 * P[j]
+                    new_p -= coeff2 * Q[j]
+                    ortho_q += abs(float(coeff1))
+                    ortho_p += abs(ortho_p)
+                normq = float(torch.sqrt(dot(new_q, new_q)))
+                if normq < __EPSILON__:
+                    next_converged = True
+                new_q /= normq
+                normp = float(dot(new_p, new_p))
+                if np.abs(normp) < __EPSILON__:
+                    next_converged = True
+                new_p /= normp
+                s_norm = c_coeff / float(dot(new_p, new_q))
+                if ortho_p < __EPSILON__ and ortho_q < __EPSILON__:
+                    break
+
+            Q.append(new_q)
+            P.append(new_p)
+            psi_q, psi_p = new_q, new_p
+            b_l.append(b_coeff)
+            c_l.append(c_coeff)
+            s_norm_l.append(s_norm)
+            if optimized and len(P) > 3:
+                P.pop(-4)
+                Q.pop(-4)
+                s_norm_l.pop(-4)
+            if verbose:
+                print("a_%d = %.8e" % (i, a_l[-1]))
+                print("b_%d = %.8e" % (i, b_l[-1]))
+                print("c_%d = %.8e" % (i, c_l[-1]))
+            self.a_coeffs, self.b_coeffs, self.c_coeffs = a_l, b_l, c_l
+            self.s_norm = s_norm_l
+            self._device_basis = (Q, P)
+            if save_dir is not None and (i + 1) % save_each == 0:
+                self.basis_Q = [q.cpu().numpy() for q in Q]
+                self.basis_P = [p.cpu().numpy() for p in P]
+                self.save_status("%s/%s_STEP%d" % (save_dir, prefix, i + 1))
+        self.a_coeffs, self.b_coeffs, self.c_coeffs = a_l, b_l, c_l
+        self.s_norm = s_norm_l
+        self.basis_Q = [q.cpu().numpy() for q in Q]
+        self.basis_P = [p.cpu().numpy() for p in P]
+        return self
+
+    # ------------------------------------------------------------------ Green function
+    def get_green_function_continued_fraction(self, w_array, use_terminator=True, last_average=1,
+                                              smearing=0):
+        """Py:4951-5047, evaluated in complex128 on the device."""
+        import torch
+        dev = self.device_ensemble.device if self.initialized else torch.device("cpu")
+        a_c = np.asarray(self.a_coeffs, dtype=np.float64)
+        b_c = np.asarray(self.b_coeffs, dtype=np.float64)
+        c_c = np.asarray(self.c_coeffs, dtype=np.float64)
+        return continued_fraction(a_c, b_c, c_c, w_array, use_terminator, last_average, smearing,
+                                  self.use_wigner, self.reverse_L, self.shift_value,
+                                  self.perturbation_modulus, device=dev)
+
+    # ------------------------------------------------------------------ status files
+    def save_abc(self, file):
+        """Py:3572-3585."""
+        total_len = len(self.c_coeffs)
+        abc = np.zeros((total_len, 3), dtype=np.double)
+        abc[:, 0] = self.a_coeffs[:total_len]
+        abc[:, 1] = self.b_coeffs
+        abc[:, 2] = self.c_coeffs
+        np.savetxt(file, abc, header="a; b; c")
+
+    def load_abc(self, file):
+        """Py:3587-3595."""
+        abc = np.loadtxt(file)
+        self.a_coeffs = abc[:, 0]
+        self.b_coeffs = abc[:, 1]
+        self.c_coeffs = abc[:, 2]
+
+    def save_status(self, file):
+        """Py:3597-3646 (same keys; sym_julia / deg_julia hold the padded blocks)."""
+        if not ".npz" in file.lower():
+            file += ".npz"
+        sym_julia, deg_julia = self._padded_symmetries()
+        np.savez_compressed(
+            file, T=self.T, nat=self.nat, m=self.m, w=self.w, pols=self.pols, n_modes=self.n_modes,
+            ignore_v3=self.ignore_v3, ignore_v4=self.ignore_v4, N=self.N, rho=self.rho, X=self.X,
+            Y=self.Y, psi=self.psi, a_coeffs=self.a_coeffs, b_coeffs=self.b_coeffs,
+            c_coeffs=self.c_coeffs, basis_Q=self.basis_Q, basis_P=self.basis_P, s_norm=self.s_norm,
+            krilov_basis=[], arnoldi_matrix=[], reverse=self.reverse_L, shift=self.shift_value,
+            perturbation_modulus=self.perturbation_modulus, use_wigner=self.use_wigner,
+            ignore_small_w=self.ignore_small_w, sym_julia=sym_julia, deg_julia=deg_julia,
+            n_syms=self.n_syms, N_degeneracy=self.N_degeneracy
+            if self.N_degeneracy is not None else np.zeros(0, dtype=np.intc))
+
+    def _padded_symmetries(self):
+        if self.symmetries is None:
+            return np.zeros(0), np.zeros(0, dtype=np.int32)
+        nb = len(self.symmetries)
+        dmax = max(s.shape[1] for s in self.symmetries)
+        sj = np.zeros((nb, self.n_syms, dmax, dmax))
+        dj = np.zeros((nb, dmax), dtype=np.int32)
+        for i, s in enumerate(self.symmetries):
+            c = s.shape[1]
+            sj[i, :, :c, :c] = s
+            dj[i, :c] = self.degenerate_space[i]
+        return sj, dj
+
+    def load_status(self, file, is_file_instance=False):
+        """Py:3649-3758. Restores the ensemble, the symmetry blocks and the Lanczos state."""
+        if not is_file_instance and not os.path.exists(file):
+            if os.path.exists(file + ".npz"):
+                file += ".npz"
+        with np.load(file, allow_pickle=False) as d:
+            data = {k: d[k] for k in d.files}
+        self.set_ensemble(data["X"], data["Y"], data["w"], data["rho"], float(data["T"]),
+                          data.get("pols"), data.get("m"))
+        self.psi = np.array(data["psi"], dtype=TYPE_DP)
+        self.a_coeffs = list(data["a_coeffs"])
+        self.b_coeffs = list(data["b_coeffs"])
+        self.c_coeffs = list(data["c_coeffs"])
+        self.basis_Q = list(data["basis_Q"])
+        self.basis_P = list(data["basis_P"])
+        self.s_norm = list(data["s_norm"])
+        self.reverse_L = bool(data["reverse"])
+        self.shift_value = float(data["shift"])
+        self.perturbation_modulus = float(data["perturbation_modulus"])
+        self.use_wigner = bool(data["use_wigner"])
+        self.ignore_v3 = bool(data["ignore_v3"])
+        self.ignore_v4 = bool(data["ignore_v4"])
+        sj, dj = data.get("sym_julia"), data.get("deg_julia")
+        if sj is not None and sj.ndim == 4 and sj.size:
+            self.set_symmetries(*blocks_from_padded(sj, dj, self.n_modes,
+                                                    data.get("N_degeneracy")))
+
+
+def blocks_from_padded(sym_julia, deg_julia, n_modes, N_degeneracy=None):
+    """sym_julia (n_blocks, N_sym, d_max, d_max) + deg_julia -> block lists (Py:753-778).
+
+    The padded form does not store the block sizes; without N_degeneracy they are recovered
+    as the largest index whose symmetry rows are not all zero.
+    """
+    nb = sym_julia.shape[0]
+    syms, degs, ndeg = [], [], []
+    for b in range(nb):
+        if N_degeneracy is not None and len(N_degeneracy) == nb:
+            d = int(N_degeneracy[b])
+        else:
+            nz = np.nonzero(np.any(sym_julia[b] != 0, axis=(0, 2)))[0]
+            d = int(nz[-1]) + 1
+        syms.append(np.ascontiguousarray(sym_julia[b, :, :d, :d]))
+        degs.append(np.ascontiguousarray(deg_julia[b, :d].astype(np.intc)))
+        ndeg.append(d)
+    bid = np.full(n_modes, -1, dtype=np.intc)
+    for b, dg in enumerate(degs):
+        bid[dg] = b
+    if (bid < 0).any():
+        raise ValueError("the padded symmetry blocks do not cover every mode")
+    return syms, np.array(ndeg, dtype=np.intc), degs, bid
+
+
+def continued_fraction(a, b, c, w_array, use_terminator=True, last_average=1, smearing=0.0,
+                       use_wigner=False, reverse_L=False, shift_value=0.0, perturbation_modulus=1.0,
+                       device=None):
+    """Continued-fraction Green function of Py:4951-5047 in complex128 (torch, any device)."""
+    import torch
+    dev = device if device is not None else torch.device("cpu")
+    w = torch.as_tensor(np.asarray(w_array, dtype=np.float64), device=dev)
+    w2 = w * w
+    cplx = torch.complex128
+    n_iters = len(a)
+    sign = -1.0 if reverse_L else 1.0
+    sym = len(c) == len(b)
+    if use_terminator:
+        a_av = float(np.mean(a[-last_average:]))
+        b_av = float(np.mean(b[-last_average:]))
+        c_av = float(np.mean(c[-last_average:])) if sym else b_av
+        aa = a_av * sign - sign * shift_value
+        bc = (b_av * sign) * (c_av * sign)
+        zz = (aa + w2) if use_wigner else (aa - w2)
+        disc = torch.sqrt((zz * zz - 4 * bc).to(cplx))
+        gf = ((zz + disc) if use_wigner else (zz - disc)) / (2 * bc)
+    else:
+        aa = float(a[-1]) * sign - sign * shift_value
+        zz = (aa + w2) if use_wigner else (aa - w2)
+        gf = 1.0 / (zz.to(cplx) + 2j * w * smearing)
+    for i in range(n_iters - 2, -1, -1):
+        aa = float(a[i]) * sign - sign * shift_value
+        bb = float(b[i]) * sign
+        cc = float(c[i]) * sign if sym else bb
+        zz = (aa + w2) if use_wigner else (aa - w2)
+        gf = 1.0 / (zz.to(cplx) + 2j * w * smearing - bb * cc * gf)
+    if not use_wigner:
+        out = gf * perturbation_modulus
+    else:
+        out = (-gf.real + 1j * gf.imag) * perturbation_modulus
+    return out.cpu().numpy()

@@ -1,0 +1,79 @@
+"""ctypes binding of libtdscha_hip.so (the C ABI declared in include/tdscha_hip.h).
+
+The library is loaded from the package's lib/ directory. There is no fallback: if the
+shared object is missing, every entry point raises.
+"""
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+TERM_F, TERM_D2V_R, TERM_D2V_Y = 1, 2, 4
+OP_TRANSPOSE, OP_WIGNER, OP_IGNORE_V3, OP_IGNORE_V4, OP_IGNORE_HARMONIC, OP_REVERSE = (
+    1, 2, 4, 8, 16, 32)
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_dpp = ctypes.POINTER(_dp)
+_ipp = ctypes.POINTER(_ip)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every symbol declared in include/tdscha_hip.h
+_TRIO = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dpp, ctypes.c_int,
+         _ip, _ipp, _ip, _dp]
+SIGNATURES = {
+    "tdscha_last_error": (ctypes.c_char_p, []),
+    "tdscha_abi_version": (ctypes.c_int, []),
+    "get_f_average_from_Y_pert_sym_fast": (None, _TRIO),
+    "get_d2v_dR2_from_R_pert_sym_fast": (None, _TRIO),
+    "get_d2v_dR2_from_Y_pert_sym_fast": (None, _TRIO),
+    "tdscha_get_f_average_from_Y_pert_sym_fast": (ctypes.c_int, _TRIO),
+    "tdscha_get_d2v_dR2_from_R_pert_sym_fast": (ctypes.c_int, _TRIO),
+    "tdscha_get_d2v_dR2_from_Y_pert_sym_fast": (ctypes.c_int, _TRIO),
+    "tdscha_get_perturb_average_sym": (ctypes.c_int, [
+        _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "tdscha_ensemble_create": (ctypes.c_int, [
+        ctypes.POINTER(_vp), ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int]),
+    "tdscha_ensemble_destroy": (ctypes.c_int, [_vp]),
+    "tdscha_ensemble_dims": (ctypes.c_int, [_vp, _ip]),
+    "tdscha_ensemble_bytes": (ctypes.c_longlong, [_vp]),
+    "tdscha_ensemble_apply": (ctypes.c_int, [
+        _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "tdscha_apply_full_L": (ctypes.c_int, [
+        _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class TdschaError(RuntimeError):
+    """An entry point of libtdscha_hip returned an error code."""
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libtdscha_hip.so is not built (%s); run __graft_entry__.build() or "
+                "python -m tdscha_amd._build" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().tdscha_last_error().decode(errors="replace")
+        if rc in (1, 2, 3, 4):
+            raise ValueError("%s: %s" % (what, msg))
+        raise TdschaError("%s failed (code %d): %s" % (what, rc, msg))

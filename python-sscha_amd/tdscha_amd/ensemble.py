@@ -1,0 +1,154 @@
+"""Device-resident ensemble: the Python face of tdscha_ensemble_* (include/tdscha_hip.h).
+
+PyTorch supplies device memory and the current HIP stream; all arithmetic happens in the
+HIP kernels of libtdscha_hip.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class SymmetryBlocks:
+    """Mode-basis symmetry blocks, as Lanczos.prepare_symmetrization leaves them
+    (Modules/DynamicalLanczos.py:741-778):
+
+    symmetries[b]       (N_sym, d_b, d_b) float64
+    N_degeneracy[b]     d_b
+    degenerate_space[b] mode ids of block b (contiguous, sorted)
+    block_id[mu]        block of mode mu
+    """
+
+    def __init__(self, symmetries, N_degeneracy, degenerate_space, block_id):
+        self.symmetries = [_c(s) for s in symmetries]
+        self.N_degeneracy = _c(N_degeneracy, np.intc)
+        self.degenerate_space = [_c(d, np.intc) for d in degenerate_space]
+        self.block_id = _c(block_id, np.intc)
+        self.n_syms = int(self.symmetries[0].shape[0])
+
+    @property
+    def n_blocks(self):
+        return len(self.symmetries)
+
+    @staticmethod
+    def identity(n):
+        return SymmetryBlocks([np.ones((1, 1, 1))] * n, np.ones(n, dtype=np.intc),
+                              [np.array([i], dtype=np.intc) for i in range(n)],
+                              np.arange(n, dtype=np.intc))
+
+    def pointer_arrays(self):
+        """(double**, int**) views; the caller keeps `self` alive during the C call."""
+        nb = self.n_blocks
+        syms = (L._dp * nb)(*[s.ctypes.data_as(L._dp) for s in self.symmetries])
+        degs = (L._ip * nb)(*[d.ctypes.data_as(L._ip) for d in self.degenerate_space])
+        return syms, degs
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is visible: the ensemble L-application runs on the GPU only")
+    return torch
+
+
+def _stream_handle(torch, device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+class DeviceEnsemble:
+    """X, Y, rho, w and the symmetry blocks uploaded once to one GPU.
+
+    precisions: iterable of "fp64", "fp32", "bf16x3" whose operand copies to build.
+    """
+
+    def __init__(self, X, Y, w, rho, T, sym: SymmetryBlocks, precisions=("fp64",), device=0,
+                 s_batch=0):
+        torch = _torch()
+        self.torch = torch
+        self.device = torch.device("cuda", device)
+        X, Y, w, rho = _c(X), _c(Y), _c(w), _c(rho)
+        N, n = X.shape
+        if Y.shape != (N, n) or w.shape != (n,) or rho.shape != (N,):
+            raise ValueError("inconsistent shapes: X %s, Y %s, w %s, rho %s"
+                             % (X.shape, Y.shape, w.shape, rho.shape))
+        self.N, self.n, self.T = N, n, float(T)
+        self.sym = sym
+        self.w = w
+        mask = 0
+        for p in precisions:
+            mask |= 1 << L.PRECISIONS[p]
+        syms, degs = sym.pointer_arrays()
+        h = ctypes.c_void_p()
+        rc = L.lib().tdscha_ensemble_create(
+            ctypes.byref(h), int(device), X.ctypes.data_as(L._dp), Y.ctypes.data_as(L._dp),
+            w.ctypes.data_as(L._dp), rho.ctypes.data_as(L._dp), float(T), n, N, sym.n_syms,
+            sym.n_blocks, syms, sym.N_degeneracy.ctypes.data_as(L._ip), degs,
+            sym.block_id.ctypes.data_as(L._ip), mask, int(s_batch))
+        L.check(rc, "tdscha_ensemble_create")
+        self._h = h
+        dims = (ctypes.c_int * 6)()
+        L.check(L.lib().tdscha_ensemble_dims(h, dims), "tdscha_ensemble_dims")
+        self.n_syms, self.s_batch, self.n_pad, self.N_pad = dims[2], dims[3], dims[4], dims[5]
+
+    @property
+    def nbytes(self):
+        return int(L.lib().tdscha_ensemble_bytes(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().tdscha_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _dev(self, a, shape):
+        t = self.torch
+        if a is None:
+            return None
+        if not isinstance(a, t.Tensor):
+            a = t.as_tensor(np.asarray(a, dtype=np.float64))
+        a = a.to(device=self.device, dtype=t.float64).contiguous()
+        if tuple(a.shape) != shape:
+            raise ValueError("expected shape %s, got %s" % (shape, tuple(a.shape)))
+        return a
+
+    def apply(self, R1, Y1, terms=L.TERM_F | L.TERM_D2V_R | L.TERM_D2V_Y, precision="fp64",
+              f=None, d2v=None, d2v_accumulate=False):
+        """Perturbed averages (f_av, d2v) as device tensors (GetPerturbAverageSym contract)."""
+        t = self.torch
+        n = self.n
+        R1 = self._dev(R1, (n,))
+        Y1 = self._dev(Y1, (n, n))
+        if f is None:
+            f = t.empty(n, dtype=t.float64, device=self.device)
+        if d2v is None:
+            d2v = t.zeros((n, n), dtype=t.float64, device=self.device)
+        ptr = lambda x: ctypes.c_void_p(x.data_ptr()) if x is not None else None
+        rc = L.lib().tdscha_ensemble_apply(self._h, L.PRECISIONS[precision], int(terms), ptr(R1),
+                                           ptr(Y1), ptr(f), ptr(d2v), int(bool(d2v_accumulate)),
+                                           _stream_handle(t, self.device))
+        L.check(rc, "tdscha_ensemble_apply")
+        return f, d2v
+
+    def apply_full_L(self, psi, flags=0, shift=0.0, precision="fp64", out=None):
+        """L psi on the device (Lanczos.apply_full_L, Modules/DynamicalLanczos.py:3467)."""
+        t = self.torch
+        n = self.n
+        psi = self._dev(psi, (n + n * (n + 1),))
+        if out is None:
+            out = t.empty_like(psi)
+        rc = L.lib().tdscha_apply_full_L(self._h, L.PRECISIONS[precision], int(flags), float(shift),
+                                         ctypes.c_void_p(psi.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()),
+                                         _stream_handle(t, self.device))
+        L.check(rc, "tdscha_apply_full_L")
+        return out

@@ -1,0 +1,247 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the golden fixtures.
+
+Tolerance (FP64 path): 1e-10, norm-wise, relative to the oracle's absolute-value sums
+(the sum of |every term|), which is the scale that bounds the rounding error of any
+summation order; for outputs that do not cancel this equals a plain relative error.
+"""
+import numpy as np
+import pytest
+
+import fixtures
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _dev_ens(ens, precisions=("fp64",), s_batch=0):
+    from tdscha_amd.ensemble import DeviceEnsemble, SymmetryBlocks
+    s = ens.sym
+    sym = SymmetryBlocks(s.symmetries, s.N_degeneracy, s.degenerate_space, s.block_id)
+    return DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sym, precisions=precisions,
+                          s_batch=s_batch)
+
+
+def _scaled_err(out, ref, scale):
+    return np.linalg.norm(out - ref) / max(np.linalg.norm(scale), 1e-300)
+
+
+def _check(ens, R1, Y1, d4, dev=None, tol=TOL64, precision="fp64"):
+    f_o, d_o, fa, da = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, d4,
+                                               ens.sym, with_abs=True)
+    dev = dev or _dev_ens(ens)
+    terms = 1 | 2 | (4 if d4 else 0)
+    f, d = dev.apply(R1, Y1, terms=terms, precision=precision)
+    f, d = f.cpu().numpy(), d.cpu().numpy()
+    ef, ed = _scaled_err(f, f_o, fa), _scaled_err(d, d_o, da)
+    print("f err %.3e  d2v err %.3e" % (ef, ed))
+    assert ef < tol and ed < tol, (ef, ed)
+    return f, d
+
+
+def test_mode10_golden_reference_vectors(mode10):
+    """The reference C trio's own outputs (compiled from CModules/LanczosFunctions.c)."""
+    kv = fixtures.load_npz("ref_kernel_vectors.npz")
+    dev = _dev_ens(mode10)
+    for k in range(int(kv["n_cases"])):
+        R1, Y1, d4 = kv["R1_%d" % k], kv["Y1_%d" % k], int(kv["d4_%d" % k])
+        _, _, fa, da = orc.perturb_average_sym(mode10.X, mode10.Y, mode10.w, mode10.rho, R1, Y1,
+                                               mode10.T, d4, mode10.sym, with_abs=True)
+        f, d = dev.apply(R1, Y1, terms=1 | 2 | (4 if d4 else 0))
+        ef = _scaled_err(f.cpu().numpy(), kv["f_%d" % k], fa)
+        ed = _scaled_err(d.cpu().numpy(), kv["d2v_%d" % k], da)
+        print("case", k, ef, ed)
+        assert ef < TOL64 and ed < TOL64
+
+
+@pytest.mark.parametrize("seed,n,N,n_sym,d4", [
+    (1, 45, 37, 12, 1),     # ragged sizes, several blocks
+    (2, 70, 130, 5, 0),     # D4 off
+    (3, 1, 3, 2, 1),        # a single mode
+    (4, 64, 64, 1, 1),      # exactly one tile, one symmetry
+    (5, 129, 65, 3, 1),     # one past the tile sizes
+])
+def test_random_ensembles(seed, n, N, n_sym, d4):
+    ens = fixtures.random_ensemble(seed, n, N, n_sym)
+    R1, Y1 = fixtures.random_perturbation(seed + 100, n, symmetric=(seed % 2 == 0))
+    _check(ens, R1, Y1, d4)
+
+
+def test_zero_temperature_and_uniform_weights():
+    ens = fixtures.random_ensemble(11, 30, 40, 4, T=0.0, weights="uniform")
+    R1, Y1 = fixtures.random_perturbation(12, 30)
+    _check(ens, R1, Y1, 1)
+
+
+def test_s_batch_independent(mode10):
+    """Results do not depend on how the symmetries are split into passes."""
+    R1, Y1 = fixtures.random_perturbation(7, 45)
+    out = []
+    for sb in (1, 7, 384):
+        f, d = _dev_ens(mode10, s_batch=sb).apply(R1, Y1)
+        out.append((f.cpu().numpy(), d.cpu().numpy()))
+    for f, d in out[1:]:
+        assert np.allclose(f, out[0][0], rtol=0, atol=1e-12 * np.abs(out[0][0]).max() + 1e-300)
+        assert np.allclose(d, out[0][1], rtol=0, atol=1e-12 * np.abs(out[0][1]).max())
+
+
+def test_trio_entry_points(mode10):
+    """The three drop-in C symbols keep the reference's overwrite / accumulate contract."""
+    import ctypes
+    from tdscha_amd import _lib as L
+    ens = mode10
+    R1, Y1 = fixtures.random_perturbation(21, 45)
+    f_o, d_o = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 1, ens.sym)
+    _, _, fa, da = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 1, ens.sym,
+                                           with_abs=True)
+    s = ens.sym
+    psyms = (L._dp * s.n_blocks)(*[a.ctypes.data_as(L._dp) for a in s.symmetries])
+    pdegs = (L._ip * s.n_blocks)(*[a.ctypes.data_as(L._ip) for a in s.degenerate_space])
+    X, Y, w, rho = (np.ascontiguousarray(a) for a in (ens.X, ens.Y, ens.w, ens.rho))
+    R1c, Y1c = np.ascontiguousarray(R1), np.ascontiguousarray(Y1)
+    common = lambda pert: (X.ctypes.data_as(L._dp), Y.ctypes.data_as(L._dp),
+                           w.ctypes.data_as(L._dp), pert.ctypes.data_as(L._dp), ens.T, 45, 10,
+                           rho.ctypes.data_as(L._dp), psyms, 384,
+                           s.N_degeneracy.ctypes.data_as(L._ip), pdegs,
+                           s.block_id.ctypes.data_as(L._ip))
+    f = np.full(45, 7.0)
+    d = np.full((45, 45), 7.0)   # garbage: must be overwritten, then accumulated into
+    lib = L.lib()
+    lib.get_f_average_from_Y_pert_sym_fast(*common(Y1c), f.ctypes.data_as(L._dp))
+    lib.get_d2v_dR2_from_R_pert_sym_fast(*common(R1c), d.ctypes.data_as(L._dp))
+    lib.get_d2v_dR2_from_Y_pert_sym_fast(*common(Y1c), d.ctypes.data_as(L._dp))
+    assert _scaled_err(f, f_o, fa) < TOL64
+    assert _scaled_err(d, d_o, da) < TOL64
+
+
+def test_get_perturb_average_sym_shim(mode10):
+    from tdscha_amd.sscha_HP_odd import GetPerturbAverageSym
+    ens = mode10
+    R1, Y1 = fixtures.random_perturbation(31, 45)
+    f_o, d_o, fa, da = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 0,
+                                               ens.sym, with_abs=True)
+    f = np.zeros(45)
+    d = np.zeros((45, 45))
+    GetPerturbAverageSym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 0, 384, ens.sym.symmetries,
+                         ens.sym.N_degeneracy, ens.sym.degenerate_space, ens.sym.block_id, f, d)
+    assert _scaled_err(f, f_o, fa) < TOL64
+    assert _scaled_err(d, d_o, da) < TOL64
+    with pytest.raises(ValueError):
+        GetPerturbAverageSym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 2, 384,
+                             ens.sym.symmetries, ens.sym.N_degeneracy, ens.sym.degenerate_space,
+                             ens.sym.block_id, f, d)
+
+
+def test_linearity_and_symmetry(mode10):
+    dev = _dev_ens(mode10)
+    R1a, Y1a = fixtures.random_perturbation(41, 45)
+    R1b, Y1b = fixtures.random_perturbation(42, 45)
+    fa, da = (x.cpu().numpy() for x in dev.apply(R1a, Y1a))
+    fb, db = (x.cpu().numpy() for x in dev.apply(R1b, Y1b))
+    fc, dc = (x.cpu().numpy() for x in dev.apply(2 * R1a - 3 * R1b, 2 * Y1a - 3 * Y1b))
+    assert np.linalg.norm(fc - (2 * fa - 3 * fb)) <= 1e-12 * np.linalg.norm(fc) + 1e-300
+    assert np.linalg.norm(dc - (2 * da - 3 * db)) <= 1e-12 * np.linalg.norm(dc)
+    assert np.array_equal(dc, dc.T) or np.abs(dc - dc.T).max() <= 1e-14 * np.abs(dc).max()
+    z, dz = (x.cpu().numpy() for x in dev.apply(np.zeros(45), np.zeros((45, 45))))
+    assert not z.any() and not dz.any()
+
+
+# ---- full L (harmonic + anharmonic) against the reference's own Krylov recurrences ----
+
+def _lanczos_from(ens, mode_kw=None):
+    from tdscha_amd import DynamicalLanczos as DL
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64)
+    lz.set_ensemble(ens.X, ens.Y, ens.w, ens.rho, ens.T)
+    s = ens.sym
+    lz.set_symmetries(s.symmetries, s.N_degeneracy, s.degenerate_space, s.block_id)
+    lz.init()
+    lz.verbose = False
+    return lz
+
+
+def test_full_L_recurrence_standard(mode10):
+    """L q_k = a_k q_k + b_k q_k+1 + c_k-1 q_k-1 on the stored basis (Py:6084-6160)."""
+    st = fixtures.load_npz("lanczos_std.npz")
+    lz = _lanczos_from(mode10)
+    Q, P, a, b, c, s = st["Q"], st["P"], st["a"], st["b"], st["c"], st["s_norm"]
+    for k in range(10):
+        Lq = lz.apply_full_L(Q[k].copy())
+        rhs = a[k] * Q[k] + b[k] * Q[k + 1] + (c[k - 1] * Q[k - 1] if k else 0)
+        err = np.linalg.norm(Lq - rhs) / np.linalg.norm(Lq)
+        Lq_o = orc.apply_full_L(mode10, Q[k])
+        err_o = np.linalg.norm(Lq - Lq_o) / np.linalg.norm(Lq_o)
+        print(k, err, err_o)
+        assert err < 1e-10 and err_o < 1e-12
+
+
+def test_full_L_transpose_matches_oracle(mode10):
+    st = fixtures.load_npz("lanczos_std.npz")
+    lz = _lanczos_from(mode10)
+    for k in (0, 3, 9):
+        p = st["P"][k]
+        out = lz.apply_full_L(p.copy(), transpose=True)
+        ref = orc.apply_full_L(mode10, p, transpose=True)
+        assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-12
+
+
+def test_full_L_wigner_and_flags(mode10):
+    st = fixtures.load_npz("lanczos_wigner.npz")
+    lz = _lanczos_from(mode10)
+    lz.use_wigner = True
+    for k in (0, 4):
+        q = st["Q"][k]
+        out = lz.apply_full_L(q.copy())
+        ref = orc.apply_full_L(mode10, q, use_wigner=True)
+        assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-12
+    lz.use_wigner = False
+    q = st["Q"][3]
+    for kw in (dict(ignore_v4=True), dict(ignore_v3=True), dict(ignore_harmonic=True),
+               dict(reverse_L=True, shift_value=1.5e-6)):
+        for key, val in kw.items():
+            setattr(lz, key, val)
+        out = lz.apply_full_L(q.copy())
+        ref = orc.apply_full_L(mode10, q, **kw)
+        assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-12, kw
+        for key in kw:
+            setattr(lz, key, type(getattr(lz, key))(0))
+
+
+def test_run_FT_coefficients_and_kat(mode10):
+    """a/b/c of the reference's 10-step run and the KAT c_4 = 6.59744344e-07
+    (tests/test_julia/test_lanczos.py:60-62)."""
+    st = fixtures.load_npz("lanczos_std.npz")
+    lz = _lanczos_from(mode10)
+    lz.prepare_mode(10)
+    lz.run_FT(10, verbose=False)
+    a, b, c = map(np.asarray, (lz.a_coeffs, lz.b_coeffs, lz.c_coeffs))
+    assert abs(c[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
+    # beyond step 7 the unreorthogonalised recurrence amplifies rounding (the fixture's own
+    # b_9 moves by 1e-2 under a change of summation order), so compare the first 8 steps
+    assert np.max(np.abs(a - st["a"]) / np.abs(st["a"])) < 1e-8
+    assert np.max(np.abs(b[:8] - st["b"][:8]) / np.abs(st["b"][:8])) < 1e-6
+    assert np.max(np.abs(c[:8] - st["c"][:8]) / np.abs(st["c"][:8])) < 1e-6
+
+
+def test_run_FT_wigner_symmetric(mode10):
+    st = fixtures.load_npz("lanczos_wigner.npz")
+    lz = _lanczos_from(mode10)
+    lz.use_wigner = True
+    lz.psi = st["Q"][0].copy()
+    lz.run_FT(10, verbose=False, run_simm=True)
+    a, b = np.asarray(lz.a_coeffs), np.asarray(lz.b_coeffs)
+    assert np.max(np.abs(a - st["a"]) / np.abs(st["a"])) < 1e-8
+    assert np.max(np.abs(b[:8] - st["b"][:8]) / np.abs(st["b"][:8])) < 1e-6
+
+
+def test_continued_fraction_device_matches_oracle(mode10):
+    st = fixtures.load_npz("lanczos_std.npz")
+    from tdscha_amd.DynamicalLanczos import continued_fraction
+    import torch
+    w = np.linspace(0, 3e-3, 257)
+    for term in (True, False):
+        g = continued_fraction(st["a"], st["b"], st["c"], w, use_terminator=term, smearing=1e-5,
+                               device=torch.device("cuda"))
+        g_o = orc.green_function_continued_fraction(st["a"], st["b"], st["c"], w,
+                                                    use_terminator=term, smearing=1e-5)
+        assert np.allclose(g, g_o, rtol=1e-12, atol=0)
