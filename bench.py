@@ -1,0 +1,177 @@
+#!/usr/bin/env python
+"""L-applications per second of the TD-SCHA ensemble L-application on one MI355X.
+
+Workload (default): synthetic rock-salt 6x6x6 supercell, 432 atoms, 1293 modes, 20000
+configurations, the full Fm-3m x translations image set (10368 symmetries), T = 300 K.
+A step is one full L application (tdscha_apply_full_L: harmonic part, pre-transform,
+the ensemble average over all 20000 x 10368 images, post-transform) on a seeded random
+unit psi, with every input already resident in HBM.
+
+Prints one JSON line. `roofline` is measured live with HIP events around every launch of
+each kernel group over the timed steps; `cpu_baseline` times the reference's own C trio
+(oracle/_ref, built from CModules/LanczosFunctions.c) on one host core over a bounded
+sample of images of the same ensemble and scales it linearly to the full image count.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "python-sscha_amd"))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (supercell L, configs, weights)
+    "rocksalt6": (6, 20000, "uniform"),
+    "rocksalt5": (5, 15000, "uniform"),
+    "rocksalt4": (4, 10000, "lognormal"),
+    "rocksalt2": (2, 2000, "uniform"),
+}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ens, seconds=12.0):
+    """Reference C trio on one core over 1 config x k symmetries, scaled to all images."""
+    import numpy as np
+    from oracle import oracle as orc
+    if not orc.ref_available():
+        return None
+    n = len(ens.w)
+    rng = np.random.default_rng(3)
+    R1 = rng.standard_normal(n)
+    A = rng.standard_normal((n, n)) * 1e-4
+    Y1 = A + A.T
+    sym = orc.SymBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    k = 1
+    t = 0.0
+    while True:
+        os.environ["OMP_NUM_THREADS"] = "1"
+        t0 = time.perf_counter()
+        orc.ref_perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 1, sym,
+                                    n_configs=1, n_syms=k)
+        t = time.perf_counter() - t0
+        if t > seconds / 4 or k >= ens.n_syms:
+            break
+        k = min(ens.n_syms, max(k * 2, int(k * (seconds / 4) / max(t, 1e-3))))
+    per_image = t / k
+    images = ens.X.shape[0] * ens.n_syms
+    return {"value": 1.0 / (per_image * images), "unit": "L-applications/s", "cores": 1,
+            "kind": "reference",
+            "sample": "reference C trio (get_f_average / get_d2v_dR2 from R / from Y, "
+                      "-O2 serial) on 1 config x %d symmetries (%.1f s, %.3g s per image), "
+                      "scaled linearly to %d configs x %d symmetries" % (
+                          k, t, per_image, ens.X.shape[0], ens.n_syms)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rocksalt6", choices=sorted(WORKLOADS))
+    ap.add_argument("--configs", type=int, default=0, help="override the number of configs")
+    ap.add_argument("--precision", default="fp64", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--s-batch", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from tdscha_amd import _lib as L
+    from tdscha_amd import synthetic as sy
+    from tdscha_amd.ensemble import DeviceEnsemble, SymmetryBlocks
+
+    dev = torch.device("cuda", 0)
+    Lc, N, weights = WORKLOADS[args.workload]
+    if args.configs:
+        N = args.configs
+    t0 = time.time()
+    ens = sy.make_ensemble(Lc, N, T=300.0, weights=weights, device=dev)
+    n = len(ens.w)
+    log("ensemble %s: n=%d N=%d N_sym=%d blocks=%d (max %d) in %.1f s" % (
+        args.workload, n, N, ens.n_syms, len(ens.N_degeneracy), max(ens.N_degeneracy),
+        time.time() - t0))
+    sym = SymmetryBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    precs = ("fp64",) if args.precision == "fp64" else ("fp64", args.precision)
+    dens = DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sym, precisions=precs,
+                          s_batch=args.s_batch)
+    log("device ensemble: %.2f GB, s_batch=%d" % (dens.nbytes / 1e9, dens.s_batch))
+    g = torch.Generator().manual_seed(3)
+    psi = torch.randn(n + n * (n + 1), generator=g, dtype=torch.float64)
+    psi = (psi / psi.norm()).to(dev)
+    out = torch.empty_like(psi)
+
+    for k in range(args.warmup):
+        dens.apply_full_L(psi, precision=args.precision, out=out)
+        torch.cuda.synchronize()
+        log("warmup %d done" % k)
+    L.lib().tdscha_profile_reset()
+    L.lib().tdscha_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        dens.apply_full_L(psi, precision=args.precision, out=out)
+        torch.cuda.synchronize()
+        log("step %d done (%.1f s)" % (k, time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    L.lib().tdscha_profile_enable(0)
+    if not torch.isfinite(out).all():
+        raise RuntimeError("non-finite output")
+
+    groups = {name: L.profile_query(gid) for gid, name in L.KGROUP_NAMES.items()}
+    log("kernel groups (ms total, launches): %s" % groups)
+    M = N * ens.n_syms
+    flop_l = 6.0 * M * n * n
+    # algorithmic flops per launch: GEMM1 carries the quadratic forms (2 N n^2 per symmetry),
+    # GEMM2 the back-accumulation (4 N n^2 per symmetry)
+    share = {"gemm1": 2.0 / 6.0, "gemm2": 4.0 / 6.0}
+    dom = max(("gemm1", "gemm2"), key=lambda k: groups[k][0])
+    ms, cnt = groups[dom]
+    per_launch_flop = flop_l * share[dom] / (cnt / args.steps)
+    achieved = per_launch_flop / (ms / cnt * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "k_%s_%s" % (dom, "f64" if args.precision == "fp64" else args.precision)}
+    value = args.steps / elapsed
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(ens)
+    res = {
+        "metric": "L-applications/sec at 432 atoms, 20k configs: BF16x3 batched vs fp64 plain",
+        "value": value,
+        "unit": "L-applications/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "rock-salt %dx%dx%d supercell, %d atoms, %d modes, %d configs, "
+                               "%d symmetry images per config, full L application (%s)" % (
+                                   Lc, Lc, Lc, 2 * Lc ** 3, n, N, ens.n_syms, args.precision),
+                   "n_modes": n, "n_configs": N, "n_syms": ens.n_syms,
+                   "flop_per_L_application": flop_l,
+                   "achieved_TFLOPs_whole_L": round(flop_l * value / 1e12, 3)},
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

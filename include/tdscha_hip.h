@@ -150,6 +150,17 @@ int tdscha_ensemble_apply(tdscha_ensemble *ens, int precision, int terms, const 
 int tdscha_apply_full_L(tdscha_ensemble *ens, int precision, int flags, double shift,
                         const double *psi, double *out, void *stream);
 
+/* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
+ * When enabled, every launch of a group is bracketed by two HIP events on its stream;
+ * query synchronises on them and returns the summed elapsed time and the launch count. */
+#define TDSCHA_KGROUP_GEMM1 0  /* quadratic forms + weights  (k_gemm1_*) */
+#define TDSCHA_KGROUP_GEMM2 1  /* back-accumulation          (k_gemm2_*) */
+#define TDSCHA_KGROUP_ROTATE 2 /* symmetry rotations         (k_rot_*, k_split_ys) */
+#define TDSCHA_KGROUP_FAV 3    /* force-average contraction  (k_fav*) */
+int tdscha_profile_enable(int on);
+int tdscha_profile_reset(void);
+int tdscha_profile_query(int group, double *total_ms, long long *count);
+
 #ifdef __cplusplus
 }
 #endif

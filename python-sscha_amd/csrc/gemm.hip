@@ -1,19 +1,24 @@
 // MFMA contractions of the ensemble L-application (gfx950).
 //
-// GEMM1 (quadratic forms + per-image weights), one workgroup per (64 configs, symmetry):
+// GEMM1 (quadratic forms + per-image weights), one workgroup per (configs tile, symmetry):
 //   Q = X Ys_s                       (N x n) . (n x n), never stored
 //   alpha_i = sum_j X_ij Q_ij,  beta_i = sum_j Ft_ij Q_ij           (row dots in the epilogue)
 //   gamma_i = X_i . g1_s,       delta_i = Ft_i . g2_s
 //   c1 = rho (k_Y alpha/8 - k_R gamma/3), c2 = rho (k_Y beta/4 - k_R delta/3)
 //   e1 = -rho alpha/6, e2 = -rho beta/3                              (LanczosFunctions.c:2195-2231,
 //                                                                     2374-2407, 2687-2725)
-// GEMM2 (back-accumulation), one workgroup per (64 x 64 output tile, symmetry):
+// GEMM2 (back-accumulation), one workgroup per (output tile, symmetry):
 //   B_s = X~^T G_s,  G_s = diag(c1) Y + diag(c2/2) X~                 (n x N) . (N x n)
 //   so that S_s (B_s + B_s^T) S_s^T = sum_i c1 (ut f^T + f ut^T) + c2 ut ut^T.
 //
-// Both are "NT" products C[m][c] = sum_k A[m][k] B[c][k] with K-contiguous operand rows,
-// staged through LDS in double-buffered BK-deep slices. FP64 uses v_mfma_f64_16x16x4f64;
-// each wave owns a 32 x 32 sub-tile (2 x 2 MFMA blocks) of the 64 x 64 workgroup tile.
+// Both are "NT" products C[m][c] = sum_k A[m][k] B[c][k] with K-contiguous operand rows.
+//
+// FP64 / FP32: v_mfma_{f64,f32}_16x16x4, 64 x 64 workgroup tile, each wave a 32 x 32
+// sub-tile (2 x 2 MFMA blocks), BK = 16 slices double-buffered through LDS as [k][m].
+// BF16x3: a*b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi with v_mfma_f32_16x16x32_bf16 and FP32
+// accumulation, 128 x 128 workgroup tile, each wave a 64 x 64 sub-tile (4 x 4 MFMA blocks),
+// BK = 32 slices staged as [m][k] rows (row stride 40 bf16: conflict-free 16-byte reads).
+// Every epilogue (row dots, weights) is FP64.
 #include "tdscha_internal.h"
 
 namespace tdscha {
@@ -21,34 +26,55 @@ namespace tdscha {
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
+template <class T> struct Acc;
+template <> struct Acc<double> { typedef v4d type; };
+template <> struct Acc<float> { typedef v4f type; };
+
+__device__ inline v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline v4f mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------- FP64 / FP32 (16x16x4)
 constexpr int BM = 64, BN = 64, BK = 16;
-constexpr int LDS_LD = BM + 16;  // row stride (doubles) of the [k][m] LDS slices
+constexpr int LDS_LD = BM + 16;  // [k][m] row stride: consecutive k rows land 16 or 32 banks apart
 
+template <class T>
 struct Gemm1Args {
-  const double *X, *Ft, *rho, *Ys, *G;
+  const T *A;        // X in the arithmetic type          [Np x npd]
+  const T *Ys;       // rotated Y1 in the arithmetic type [sb x npd x npd]
+  const double *X, *Ft, *rho, *G;
   double *W;
-  int Np, npd, n;
-  int do_q;              // compute the Y1 quadratic forms (alpha, beta)
-  double kf, kr, ky;     // which reference terms are requested (0 or 1)
+  int Np, npd;
+  int do_q;
+  double kf, kr, ky;
 };
 
+template <class T>
 struct Gemm2Args {
-  const double *XtT, *YT, *W;
+  const T *XtT, *YT;  // in the arithmetic type [npd x Np]
+  const double *W;
   double *B;
   int Np, npd;
 };
 
-// Stage a 64 x BK slice of a row-major matrix (rows r0.., cols k0..) into LDS as [k][row].
-__device__ inline void load_slice(const double *__restrict__ M, int ld, int r0, int k0, double (&reg)[4]) {
+// Stage a 64 x BK slice of a row-major matrix (rows r0.., cols k0..) into registers.
+template <class T>
+__device__ inline void load_slice(const T *__restrict__ M, int ld, int r0, int k0, T (&reg)[4]) {
   const int t = threadIdx.x;
   const int row = t >> 2, kk = (t & 3) * 4;
-  const double2 *p = reinterpret_cast<const double2 *>(M + (size_t)(r0 + row) * ld + k0 + kk);
-  double2 v0 = p[0], v1 = p[1];
-  reg[0] = v0.x; reg[1] = v0.y; reg[2] = v1.x; reg[3] = v1.y;
+  const T *p = M + (size_t)(r0 + row) * ld + k0 + kk;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) reg[j] = p[j];
 }
 
-__device__ inline void store_slice(double *lds, const double (&reg)[4]) {
+template <class T>
+__device__ inline void store_slice(T *lds, const T (&reg)[4]) {
   const int t = threadIdx.x;
   const int row = t >> 2, kk = (t & 3) * 4;
 #pragma unroll
@@ -56,26 +82,39 @@ __device__ inline void store_slice(double *lds, const double (&reg)[4]) {
 }
 
 // 16 MFMAs of one BK slice for this wave's 2 x 2 block of 16 x 16 tiles.
-__device__ inline void mma_slice(const double *As, const double *Bs, int wr, int wc, v4d (&acc)[2][2]) {
+template <class T, class V>
+__device__ inline void mma_slice(const T *As, const T *Bs, int wr, int wc, V (&acc)[2][2]) {
   const int l = threadIdx.x & 63;
   const int lr = l & 15, lk = l >> 4;
 #pragma unroll
   for (int kq = 0; kq < BK / 4; ++kq) {
     const int k = kq * 4 + lk;
-    double a0 = As[k * LDS_LD + wr * 32 + lr];
-    double a1 = As[k * LDS_LD + wr * 32 + 16 + lr];
-    double b0 = Bs[k * LDS_LD + wc * 32 + lr];
-    double b1 = Bs[k * LDS_LD + wc * 32 + 16 + lr];
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    T a0 = As[k * LDS_LD + wr * 32 + lr];
+    T a1 = As[k * LDS_LD + wr * 32 + 16 + lr];
+    T b0 = Bs[k * LDS_LD + wc * 32 + lr];
+    T b1 = Bs[k * LDS_LD + wc * 32 + 16 + lr];
+    acc[0][0] = mfma(a0, b0, acc[0][0]);
+    acc[0][1] = mfma(a0, b1, acc[0][1]);
+    acc[1][0] = mfma(a1, b0, acc[1][0]);
+    acc[1][1] = mfma(a1, b1, acc[1][1]);
   }
 }
 
-__global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
-  __shared__ double As[2][BK * LDS_LD];
-  __shared__ double Bs[2][BK * LDS_LD];
+// Per-image weights from the four row sums (shared by every precision).
+__device__ inline void write_weights(double *W, int Np, int i, double rho, double alpha,
+                                     double beta, double gamma, double delta, double kf,
+                                     double kr, double ky) {
+  W[i] = rho * (ky * alpha / 8 - kr * gamma / 3);
+  W[(size_t)Np + i] = rho * (ky * beta / 4 - kr * delta / 3);
+  W[2 * (size_t)Np + i] = -kf * rho * alpha / 6;
+  W[3 * (size_t)Np + i] = -kf * rho * beta / 3;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_gemm1(Gemm1Args<T> g) {
+  typedef typename Acc<T>::type V;
+  __shared__ T As[2][BK * LDS_LD];
+  __shared__ T Bs[2][BK * LDS_LD];
   __shared__ double red[2][BM][4];
 
   const int i0 = blockIdx.x * BM;
@@ -83,11 +122,10 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = w >> 1, wc = w & 1;
   const int lr = l & 15, lg = l >> 4;
-  const double *Ys = g.Ys + (size_t)sl * g.npd * g.npd;
+  const T *Ys = g.Ys + (size_t)sl * g.npd * g.npd;
   const double *g1 = g.G + (size_t)sl * 2 * g.npd;
   const double *g2 = g1 + g.npd;
 
-  // per-lane partial row sums: rows wr*32 + ti*16 + 4*lg + r
   double pa[2][4], pb[2][4], pc[2][4], pd[2][4];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
@@ -96,15 +134,17 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
 
   const int KT = g.npd / BK;
   for (int c0 = 0; c0 < g.npd; c0 += BN) {
-    v4d acc[2][2];
+    V acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
 
     if (g.do_q) {
-      double ra[4], rb[4];
-      load_slice(g.X, g.npd, i0, 0, ra);
+      T ra[4], rb[4];
+      load_slice(g.A, g.npd, i0, 0, ra);
       load_slice(Ys, g.npd, c0, 0, rb);
       store_slice(As[0], ra);
       store_slice(Bs[0], rb);
@@ -112,7 +152,7 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
       for (int kt = 0; kt < KT; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < KT) {
-          load_slice(g.X, g.npd, i0, (kt + 1) * BK, ra);
+          load_slice(g.A, g.npd, i0, (kt + 1) * BK, ra);
           load_slice(Ys, g.npd, c0, (kt + 1) * BK, rb);
         }
         mma_slice(As[cur], Bs[cur], wr, wc, acc);
@@ -124,7 +164,6 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
       }
     }
 
-    // epilogue: row dots of this column tile
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj) {
       const int c = c0 + wc * 32 + tj * 16 + lr;
@@ -136,7 +175,7 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
           const int m = wr * 32 + ti * 16 + 4 * lg + r;
           const size_t o = (size_t)(i0 + m) * g.npd + c;
           const double x = g.X[o], ft = g.Ft[o];
-          const double q = acc[ti][tj][r];
+          const double q = (double)acc[ti][tj][r];
           pa[ti][r] = fma(x, q, pa[ti][r]);
           pb[ti][r] = fma(ft, q, pb[ti][r]);
           pc[ti][r] = fma(x, gg1, pc[ti][r]);
@@ -146,7 +185,6 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
     }
   }
 
-  // reduce over the 16 lanes holding the same rows (different columns)
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
@@ -169,40 +207,31 @@ __global__ __launch_bounds__(256) void k_gemm1_f64(Gemm1Args g) {
   __syncthreads();
   if (threadIdx.x < BM) {
     const int m = threadIdx.x, i = i0 + m;
-    const double alpha = red[0][m][0] + red[1][m][0];
-    const double beta = red[0][m][1] + red[1][m][1];
-    const double gamma = red[0][m][2] + red[1][m][2];
-    const double delta = red[0][m][3] + red[1][m][3];
-    const double rho = g.rho[i];
-    double *Ws = g.W + (size_t)sl * 4 * g.Np;
-    Ws[i] = rho * (g.ky * alpha / 8 - g.kr * gamma / 3);
-    Ws[(size_t)g.Np + i] = rho * (g.ky * beta / 4 - g.kr * delta / 3);
-    Ws[2 * (size_t)g.Np + i] = -g.kf * rho * alpha / 6;
-    Ws[3 * (size_t)g.Np + i] = -g.kf * rho * beta / 3;
+    write_weights(g.W + (size_t)sl * 4 * g.Np, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
+                  red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
+                  red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
   }
 }
 
-// Stage the computed operand G^T[c][k] = c1[k] YT[c][k] + c2[k]/2 XtT[c][k].
-__device__ inline void load_g_slice(const Gemm2Args &g, const double *c1, const double *c2, int r0,
-                                    int k0, double (&reg)[4]) {
+// The computed operand G^T[c][k] = c1[k] YT[c][k] + c2[k]/2 XtT[c][k].
+template <class T>
+__device__ inline void load_g_slice(const Gemm2Args<T> &g, const double *c1, const double *c2,
+                                    int r0, int k0, T (&reg)[4]) {
   const int t = threadIdx.x;
   const int row = t >> 2, kk = (t & 3) * 4;
   const size_t o = (size_t)(r0 + row) * g.Np + k0 + kk;
-  const double2 *py = reinterpret_cast<const double2 *>(g.YT + o);
-  const double2 *px = reinterpret_cast<const double2 *>(g.XtT + o);
-  const double2 *pc1 = reinterpret_cast<const double2 *>(c1 + k0 + kk);
-  const double2 *pc2 = reinterpret_cast<const double2 *>(c2 + k0 + kk);
-  double2 y0 = py[0], y1 = py[1], x0 = px[0], x1 = px[1];
-  double2 a0 = pc1[0], a1 = pc1[1], b0 = pc2[0], b1 = pc2[1];
-  reg[0] = fma(a0.x, y0.x, 0.5 * b0.x * x0.x);
-  reg[1] = fma(a0.y, y0.y, 0.5 * b0.y * x0.y);
-  reg[2] = fma(a1.x, y1.x, 0.5 * b1.x * x1.x);
-  reg[3] = fma(a1.y, y1.y, 0.5 * b1.y * x1.y);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const T a = (T)c1[k0 + kk + j], b = (T)(0.5 * c2[k0 + kk + j]);
+    reg[j] = a * g.YT[o + j] + b * g.XtT[o + j];
+  }
 }
 
-__global__ __launch_bounds__(256) void k_gemm2_f64(Gemm2Args g) {
-  __shared__ double As[2][BK * LDS_LD];
-  __shared__ double Bs[2][BK * LDS_LD];
+template <class T>
+__global__ __launch_bounds__(256) void k_gemm2(Gemm2Args<T> g) {
+  typedef typename Acc<T>::type V;
+  __shared__ T As[2][BK * LDS_LD];
+  __shared__ T Bs[2][BK * LDS_LD];
   const int m0 = blockIdx.x * BM, c0 = blockIdx.y * BN, sl = blockIdx.z;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = w >> 1, wc = w & 1;
@@ -210,14 +239,16 @@ __global__ __launch_bounds__(256) void k_gemm2_f64(Gemm2Args g) {
   const double *c1 = g.W + (size_t)sl * 4 * g.Np;
   const double *c2 = c1 + g.Np;
 
-  v4d acc[2][2];
+  V acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = 0;
 
   const int KT = g.Np / BK;
-  double ra[4], rb[4];
+  T ra[4], rb[4];
   load_slice(g.XtT, g.Np, m0, 0, ra);
   load_g_slice(g, c1, c2, c0, 0, rb);
   store_slice(As[0], ra);
@@ -246,32 +277,309 @@ __global__ __launch_bounds__(256) void k_gemm2_f64(Gemm2Args g) {
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wr * 32 + ti * 16 + 4 * lg + r;
         const int c = c0 + wc * 32 + tj * 16 + lr;
-        Bo[(size_t)m * g.npd + c] = acc[ti][tj][r];
+        Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
+      }
+}
+
+// ---------------------------------------------------------------- BF16 x3 (16x16x32)
+constexpr int LB = 128;      // workgroup tile (rows and columns)
+constexpr int LK = 32;       // k slice
+constexpr int LLD = LK + 8;  // LDS row stride in bf16 (80 bytes)
+
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+struct Gemm1BfArgs {
+  const uint16_t *Xh, *Xl;    // [Np x npd]
+  const uint16_t *Ysh, *Ysl;  // [sb x npd x npd]
+  const double *X, *Ft, *rho, *G;
+  double *W;
+  int Np, npd;
+  int do_q;
+  double kf, kr, ky;
+};
+
+struct Gemm2BfArgs {
+  const uint16_t *XtTh, *XtTl;  // [npd x Np]
+  const float *YTf, *XtTf;      // [npd x Np]
+  const double *W;
+  double *B;
+  int Np, npd;
+};
+
+// Two 16-byte chunks per thread cover a 128 x 32 bf16 slice: rows t/4 and t/4 + 64.
+__device__ inline void load_bf_slice(const uint16_t *__restrict__ M, int ld, int rows, int r0,
+                                     int k0, uint4 (&reg)[2]) {
+  const int t = threadIdx.x;
+  const int row = t >> 2, ch = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r = r0 + row + 64 * h;
+    reg[h] = (r < rows) ? *reinterpret_cast<const uint4 *>(M + (size_t)r * ld + k0 + ch)
+                        : make_uint4(0, 0, 0, 0);
+  }
+}
+
+__device__ inline void store_bf_slice(uint16_t *lds, const uint4 (&reg)[2]) {
+  const int t = threadIdx.x;
+  const int row = t >> 2, ch = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4 *>(lds + (row + 64 * h) * LLD + ch) = reg[h];
+}
+
+__device__ inline v8bf lds_frag(const uint16_t *lds, int row, int kg) {
+  uint4 u = *reinterpret_cast<const uint4 *>(lds + row * LLD + kg * 8);
+  return __builtin_bit_cast(v8bf, u);
+}
+
+// 48 MFMAs of one 32-deep slice for this wave's 4 x 4 block of 16 x 16 tiles.
+__device__ inline void mma_bf16x3(const uint16_t *Ah, const uint16_t *Al, const uint16_t *Bh,
+                                  const uint16_t *Bl, int wr, int wc, v4f (&acc)[4][4]) {
+  const int l = threadIdx.x & 63;
+  const int lr = l & 15, lg = l >> 4;
+  v8bf bh[4], bl[4];
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj) {
+    bh[tj] = lds_frag(Bh, wc * 64 + tj * 16 + lr, lg);
+    bl[tj] = lds_frag(Bl, wc * 64 + tj * 16 + lr, lg);
+  }
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti) {
+    const v8bf ah = lds_frag(Ah, wr * 64 + ti * 16 + lr, lg);
+    const v8bf al = lds_frag(Al, wr * 64 + ti * 16 + lr, lg);
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) {
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gemm1_bf16x3(Gemm1BfArgs g) {
+  __shared__ uint16_t Ah[LB * LLD], Al[LB * LLD], Bh[LB * LLD], Bl[LB * LLD];
+  __shared__ double red[2][LB][4];
+  const int i0 = blockIdx.x * LB;
+  const int sl = blockIdx.y;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1;
+  const int lr = l & 15, lg = l >> 4;
+  const size_t ys_off = (size_t)sl * g.npd * g.npd;
+  const double *g1 = g.G + (size_t)sl * 2 * g.npd;
+  const double *g2 = g1 + g.npd;
+
+  for (int t = threadIdx.x; t < 2 * LB * 4; t += 256) (&red[0][0][0])[t] = 0.0;
+  __syncthreads();
+
+  const int KT = g.npd / LK;
+  for (int c0 = 0; c0 < g.npd; c0 += LB) {
+    v4f acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+    if (g.do_q) {
+      uint4 rah[2], ral[2], rbh[2], rbl[2];
+      load_bf_slice(g.Xh, g.npd, g.Np, i0, 0, rah);
+      load_bf_slice(g.Xl, g.npd, g.Np, i0, 0, ral);
+      load_bf_slice(g.Ysh + ys_off, g.npd, g.npd, c0, 0, rbh);
+      load_bf_slice(g.Ysl + ys_off, g.npd, g.npd, c0, 0, rbl);
+      for (int kt = 0; kt < KT; ++kt) {
+        __syncthreads();
+        store_bf_slice(Ah, rah);
+        store_bf_slice(Al, ral);
+        store_bf_slice(Bh, rbh);
+        store_bf_slice(Bl, rbl);
+        __syncthreads();
+        if (kt + 1 < KT) {
+          const int k0 = (kt + 1) * LK;
+          load_bf_slice(g.Xh, g.npd, g.Np, i0, k0, rah);
+          load_bf_slice(g.Xl, g.npd, g.Np, i0, k0, ral);
+          load_bf_slice(g.Ysh + ys_off, g.npd, g.npd, c0, k0, rbh);
+          load_bf_slice(g.Ysl + ys_off, g.npd, g.npd, c0, k0, rbl);
+        }
+        mma_bf16x3(Ah, Al, Bh, Bl, wr, wc, acc);
+      }
+    }
+    // epilogue: FP64 row dots of this column tile, reduced over the 16 column lanes
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti) {
+      double p[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj) {
+        const int c = c0 + wc * 64 + tj * 16 + lr;
+        if (c < g.npd) {
+          const double gg1 = g1[c], gg2 = g2[c];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int m = wr * 64 + ti * 16 + 4 * lg + r;
+            const size_t o = (size_t)(i0 + m) * g.npd + c;
+            const double x = g.X[o], ft = g.Ft[o];
+            const double q = (double)acc[ti][tj][r];
+            p[r][0] = fma(x, q, p[r][0]);
+            p[r][1] = fma(ft, q, p[r][1]);
+            p[r][2] = fma(x, gg1, p[r][2]);
+            p[r][3] = fma(ft, gg2, p[r][3]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double v = p[r][q];
+#pragma unroll
+          for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+          if (lr == 0) red[wc][wr * 64 + ti * 16 + 4 * lg + r][q] += v;
+        }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < LB) {
+    const int m = threadIdx.x, i = i0 + m;
+    write_weights(g.W + (size_t)sl * 4 * g.Np, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
+                  red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
+                  red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
+  }
+}
+
+// G^T slice (rows c0.., k0..k0+31) formed in FP32 and split into bf16 hi / lo.
+__device__ inline void load_g_bf_slice(const Gemm2BfArgs &g, const double *c1, const double *c2,
+                                       int r0, int k0, uint4 (&rh)[2], uint4 (&rl)[2]) {
+  const int t = threadIdx.x;
+  const int row = t >> 2, ch = (t & 3) * 8;
+  float a[8], b[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    a[j] = (float)c1[k0 + ch + j];
+    b[j] = (float)(0.5 * c2[k0 + ch + j]);
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r = r0 + row + 64 * h;
+    uint16_t hi[8], lo[8];
+    if (r < g.npd) {
+      const float4 *py = reinterpret_cast<const float4 *>(g.YTf + (size_t)r * g.Np + k0 + ch);
+      const float4 *px = reinterpret_cast<const float4 *>(g.XtTf + (size_t)r * g.Np + k0 + ch);
+      float4 y0 = py[0], y1 = py[1], x0 = px[0], x1 = px[1];
+      float yv[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+      float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float v = fmaf(a[j], yv[j], b[j] * xv[j]);
+        hi[j] = bf16_rne(v);
+        lo[j] = bf16_rne(v - bf16_to_f(hi[j]));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) hi[j] = lo[j] = 0;
+    }
+    rh[h] = make_uint4(hi[0] | ((uint32_t)hi[1] << 16), hi[2] | ((uint32_t)hi[3] << 16),
+                       hi[4] | ((uint32_t)hi[5] << 16), hi[6] | ((uint32_t)hi[7] << 16));
+    rl[h] = make_uint4(lo[0] | ((uint32_t)lo[1] << 16), lo[2] | ((uint32_t)lo[3] << 16),
+                       lo[4] | ((uint32_t)lo[5] << 16), lo[6] | ((uint32_t)lo[7] << 16));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gemm2_bf16x3(Gemm2BfArgs g) {
+  __shared__ uint16_t Ah[LB * LLD], Al[LB * LLD], Bh[LB * LLD], Bl[LB * LLD];
+  const int m0 = blockIdx.x * LB, c0 = blockIdx.y * LB, sl = blockIdx.z;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1;
+  const int lr = l & 15, lg = l >> 4;
+  const double *c1 = g.W + (size_t)sl * 4 * g.Np;
+  const double *c2 = c1 + g.Np;
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int KT = g.Np / LK;
+  uint4 rah[2], ral[2], rbh[2], rbl[2];
+  load_bf_slice(g.XtTh, g.Np, g.npd, m0, 0, rah);
+  load_bf_slice(g.XtTl, g.Np, g.npd, m0, 0, ral);
+  load_g_bf_slice(g, c1, c2, c0, 0, rbh, rbl);
+  for (int kt = 0; kt < KT; ++kt) {
+    __syncthreads();
+    store_bf_slice(Ah, rah);
+    store_bf_slice(Al, ral);
+    store_bf_slice(Bh, rbh);
+    store_bf_slice(Bl, rbl);
+    __syncthreads();
+    if (kt + 1 < KT) {
+      const int k0 = (kt + 1) * LK;
+      load_bf_slice(g.XtTh, g.Np, g.npd, m0, k0, rah);
+      load_bf_slice(g.XtTl, g.Np, g.npd, m0, k0, ral);
+      load_g_bf_slice(g, c1, c2, c0, k0, rbh, rbl);
+    }
+    mma_bf16x3(Ah, Al, Bh, Bl, wr, wc, acc);
+  }
+  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wr * 64 + ti * 16 + 4 * lg + r;
+        const int c = c0 + wc * 64 + tj * 16 + lr;
+        if (m < g.npd && c < g.npd) Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
       }
 }
 
 }  // namespace
 
 void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st) {
-  Gemm1Args g;
-  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.Ys = e.Ys; g.G = e.G; g.W = e.W;
-  g.Np = e.Np; g.npd = e.npd; g.n = e.n;
-  g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
-  g.kf = (what & kTermF) ? 1.0 : 0.0;
-  g.kr = (what & kTermD2vR) ? 1.0 : 0.0;
-  g.ky = (what & kTermD2vY) ? 1.0 : 0.0;
-  (void)prec;
-  dim3 grid(e.Np / BM, sb);
-  hipLaunchKernelGGL(k_gemm1_f64, grid, dim3(256), 0, st, g);
+  const int do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  const double kf = (what & kTermF) ? 1.0 : 0.0;
+  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
+  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
+  if (prec == kBF16x3) {
+    Gemm1BfArgs g;
+    const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+    g.Xh = (const uint16_t *)e.X_hi; g.Xl = (const uint16_t *)e.X_lo;
+    g.Ysh = (const uint16_t *)e.Ys_lp; g.Ysl = (const uint16_t *)e.Ys_lp + plane;
+    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
+    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
+    hipLaunchKernelGGL(k_gemm1_bf16x3, dim3(e.Np / LB, sb), dim3(256), 0, st, g);
+  } else if (prec == kFP32) {
+    Gemm1Args<float> g;
+    g.A = e.X_f; g.Ys = (const float *)e.Ys_lp;
+    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
+    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
+    hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
+  } else {
+    Gemm1Args<double> g;
+    g.A = e.X; g.Ys = e.Ys;
+    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
+    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
+    hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
+  }
 }
 
 void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st) {
-  Gemm2Args g;
-  g.XtT = e.XtT; g.YT = e.YT; g.W = e.W; g.B = e.B;
-  g.Np = e.Np; g.npd = e.npd;
-  (void)prec;
-  dim3 grid(e.npd / BM, e.npd / BN, sb);
-  hipLaunchKernelGGL(k_gemm2_f64, grid, dim3(256), 0, st, g);
+  if (prec == kBF16x3) {
+    Gemm2BfArgs g;
+    g.XtTh = (const uint16_t *)e.XtT_hi; g.XtTl = (const uint16_t *)e.XtT_lo;
+    g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
+    const int t = (e.npd + LB - 1) / LB;
+    hipLaunchKernelGGL(k_gemm2_bf16x3, dim3(t, t, sb), dim3(256), 0, st, g);
+  } else if (prec == kFP32) {
+    Gemm2Args<float> g;
+    g.XtT = e.XtT_f; g.YT = e.YT_f; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
+    hipLaunchKernelGGL(k_gemm2<float>, dim3(e.npd / BM, e.npd / BN, sb), dim3(256), 0, st, g);
+  } else {
+    Gemm2Args<double> g;
+    g.XtT = e.XtT; g.YT = e.YT; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
+    hipLaunchKernelGGL(k_gemm2<double>, dim3(e.npd / BM, e.npd / BN, sb), dim3(256), 0, st, g);
+  }
 }
 
 }  // namespace tdscha

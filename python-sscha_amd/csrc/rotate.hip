@@ -195,21 +195,26 @@ __global__ void k_finalize(const double *__restrict__ f_acc, const double *__res
   if (do_f && a == 0) f_out[b] = f_acc[b] * inv_norm;
 }
 
-// Ys (fp64) -> bf16 (hi, lo) planes or fp32, same [sb x npd x npd] layout.
-__global__ void k_split_ys(const double *__restrict__ Ys, void *out, size_t count, int prec) {
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// fp64 -> bf16 (hi, lo) planes (hi at out, lo at out + count) or fp32, element for element.
+// hi = RNE(v), lo = RNE(v - hi) with v rounded to fp32 first: hi + lo keeps 16 bits.
+__global__ void k_convert(const double *__restrict__ src, void *out, size_t count, int prec) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < count; i += stride) {
-    double v = Ys[i];
+    const float f = (float)src[i];
     if (prec == kBF16x3) {
-      __hip_bfloat16 *hi = (__hip_bfloat16 *)out;
-      __hip_bfloat16 *lo = hi + count;
-      float f = (float)v;
-      __hip_bfloat16 h = __float2bfloat16(f);
+      uint16_t *hi = (uint16_t *)out;
+      const uint16_t h = bf16_rne(f);
       hi[i] = h;
-      lo[i] = __float2bfloat16(f - __bfloat162float(h));
+      hi[count + i] = bf16_rne(f - __uint_as_float((uint32_t)h << 16));
     } else {
-      ((float *)out)[i] = (float)v;
+      ((float *)out)[i] = f;
     }
   }
 }
@@ -275,8 +280,19 @@ void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f
 }
 
 void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st) {
-  size_t count = (size_t)sb * e.npd * e.npd;
-  hipLaunchKernelGGL(k_split_ys, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, count, prec);
+  // the planes are laid out for s_batch symmetries (the GEMM1 hi / lo offsets assume it)
+  const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+  const size_t count = (size_t)sb * e.npd * e.npd;
+  if (prec == kBF16x3) {
+    // hi plane [0, plane), lo plane [plane, 2 plane): convert hi and lo of the first `count`
+    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, plane, prec);
+  } else {
+    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, count, prec);
+  }
+}
+
+void launch_convert(const double *src, void *dst, size_t count, int prec, hipStream_t st) {
+  hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, src, dst, count, prec);
 }
 
 }  // namespace tdscha

@@ -123,6 +123,39 @@ int choose_s_batch(const Ensemble &e, int requested) {
   return (int)sb;
 }
 
+// Optional event timing of the kernel groups (tdscha_profile_*): every launch of a group
+// is bracketed by a pair of events on the launch stream; elapsed times are read back
+// (after a synchronisation) by tdscha_profile_query.
+struct Profiler {
+  bool on = false;
+  struct Rec { int group; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+};
+Profiler g_prof;
+
+struct Span {
+  int group;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  Span(int g, hipStream_t s) : group(g), st(s) {
+    if (g_prof.on) { a = g_prof.get(); (void)hipEventRecord(a, st); }
+  }
+  ~Span() {
+    if (g_prof.on && a) {
+      hipEvent_t b = g_prof.get();
+      (void)hipEventRecord(b, st);
+      g_prof.recs.push_back({group, a, b});
+    }
+  }
+};
+
 // Per-pass orchestration of one ensemble application. Y1/R1 must already be staged in
 // e.Ysym / e.R1p (prep or unpack kernel). Leaves unnormalised sums in f_acc / d2v_acc.
 int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
@@ -135,19 +168,30 @@ int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
   if (!need_r) launch_zero(e.G, (size_t)e.s_batch * 2 * e.npd, st);
   for (int s0 = 0; s0 < e.N_sym; s0 += e.s_batch) {
     const int sb = (e.N_sym - s0) < e.s_batch ? (e.N_sym - s0) : e.s_batch;
-    if (need_q) {
-      launch_rot_right(e, e.Ysym, 0, false, e.Tw, s0, sb, false, st);
-      launch_rot_left(e, e.Tw, e.Ys, s0, sb, true, false, st);
-      if (prec != kFP64) launch_split_ys(e, sb, prec, st);
+    {
+      Span sp(TDSCHA_KGROUP_ROTATE, st);
+      if (need_q) {
+        launch_rot_right(e, e.Ysym, 0, false, e.Tw, s0, sb, false, st);
+        launch_rot_left(e, e.Tw, e.Ys, s0, sb, true, false, st);
+        if (prec != kFP64) launch_split_ys(e, sb, prec, st);
+      }
+      if (need_r) launch_rot_vec(e, s0, sb, st);
     }
-    if (need_r) launch_rot_vec(e, s0, sb, st);
-    launch_gemm1(e, sb, prec, what, st);
+    {
+      Span sp(TDSCHA_KGROUP_GEMM1, st);
+      launch_gemm1(e, sb, prec, what, st);
+    }
     if (need_d) {
-      launch_gemm2(e, sb, prec, st);
+      {
+        Span sp(TDSCHA_KGROUP_GEMM2, st);
+        launch_gemm2(e, sb, prec, st);
+      }
+      Span sp(TDSCHA_KGROUP_ROTATE, st);
       launch_rot_right(e, e.B, (long)e.npd * e.npd, true, e.Tw, s0, sb, true, st);
       launch_rot_left(e, e.Tw, e.d2v_acc, s0, sb, false, true, st);
     }
     if (need_f) {
+      Span sp(TDSCHA_KGROUP_FAV, st);
       launch_fav(e, sb, st);
       launch_fav_rot_acc(e, s0, sb, st);
     }
@@ -169,10 +213,6 @@ int check_prec(const Ensemble &e, int prec) {
     set_error("ensemble was created without the operand copies of precision " + std::to_string(prec));
     return 4;
   }
-  if (prec != kFP64) {
-    set_error("precision " + std::to_string(prec) + " is not available in this build");
-    return 4;
-  }
   return 0;
 }
 
@@ -182,6 +222,41 @@ extern "C" {
 
 const char *tdscha_last_error(void) { return g_err.c_str(); }
 int tdscha_abi_version(void) { return TDSCHA_ABI_VERSION; }
+
+int tdscha_profile_enable(int on) {
+  g_prof.on = on != 0;
+  return 0;
+}
+
+int tdscha_profile_reset(void) {
+  for (auto &r : g_prof.recs) {
+    g_prof.pool.push_back(r.a);
+    g_prof.pool.push_back(r.b);
+  }
+  g_prof.recs.clear();
+  return 0;
+}
+
+int tdscha_profile_query(int group, double *total_ms, long long *count) {
+  if (!total_ms || !count) { set_error("NULL argument"); return 1; }
+  double tot = 0.0;
+  long long cnt = 0;
+  for (auto &r : g_prof.recs) {
+    if (r.group != group) continue;
+    float ms = 0.0f;
+    hipError_t err = hipEventSynchronize(r.b);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, r.a, r.b);
+    if (err != hipSuccess) {
+      set_error(std::string("event timing failed: ") + hipGetErrorString(err));
+      return 10;
+    }
+    tot += ms;
+    ++cnt;
+  }
+  *total_ms = tot;
+  *count = cnt;
+  return 0;
+}
 
 int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, const double *Y,
                            const double *w, const double *rho, double T, int n_modes, int n_configs,
@@ -273,6 +348,22 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= dev_alloc(h, &e.V, sb * npd);
   rc |= dev_alloc(h, &e.d2v_acc, nn);
   rc |= dev_alloc(h, &e.f_acc, (size_t)npd);
+  const bool want32 = e.prec_mask & (1 << kFP32), wantbf = e.prec_mask & (1 << kBF16x3);
+  const size_t NX = (size_t)Np * npd;
+  if (want32 || wantbf) {
+    rc |= dev_alloc(h, &e.XtT_f, NX);
+    rc |= dev_alloc(h, &e.YT_f, NX);
+    char *lp = nullptr;
+    rc |= dev_alloc(h, &lp, sb * nn * 4);  // fp32 plane or bf16 hi + lo planes
+    e.Ys_lp = lp;
+  }
+  if (want32) rc |= dev_alloc(h, &e.X_f, NX);
+  if (wantbf) {
+    rc |= dev_alloc(h, &e.X_hi, 2 * NX);
+    rc |= dev_alloc(h, &e.XtT_hi, 2 * NX);
+    e.X_lo = e.X_hi + NX;
+    e.XtT_lo = e.XtT_hi + NX;
+  }
   if (rc) { tdscha_ensemble_destroy(h); return 11; }
 
   auto up = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -303,6 +394,23 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
     }
   }
   if (rc) { tdscha_ensemble_destroy(h); return rc; }
+  // reduced-precision operand copies, converted on the device from the fp64 uploads
+  if (want32 || wantbf) {
+    launch_convert(e.XtT, e.XtT_f, NX, kFP32, nullptr);
+    launch_convert(e.YT, e.YT_f, NX, kFP32, nullptr);
+  }
+  if (want32) launch_convert(e.X, e.X_f, NX, kFP32, nullptr);
+  if (wantbf) {
+    launch_convert(e.X, e.X_hi, NX, kBF16x3, nullptr);
+    launch_convert(e.XtT, e.XtT_hi, NX, kBF16x3, nullptr);
+  }
+  hipError_t err = hipDeviceSynchronize();
+  if (err == hipSuccess) err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_error(std::string("operand conversion failed: ") + hipGetErrorString(err));
+    tdscha_ensemble_destroy(h);
+    return 10;
+  }
   *out = h;
   return 0;
 }

@@ -22,7 +22,7 @@
 
 namespace tdscha {
 
-constexpr int kPadRows = 64;   // config padding (GEMM1 row tile, GEMM2 k tile)
+constexpr int kPadRows = 128;  // config padding (GEMM1 row tile, GEMM2 k tile)
 constexpr int kPadCols = 64;   // mode padding (GEMM tiles along modes)
 constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 
@@ -55,8 +55,8 @@ struct Ensemble {
   double *fups = nullptr, *fpsi = nullptr, *S = nullptr;
   double *w = nullptr, *nocc = nullptr;   // frequencies [n] and Bose occupations [n]
   ModeMeta *meta = nullptr;
-  // reduced-precision operand copies
-  __hip_bfloat16 *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
+  // reduced-precision operand copies (bf16 stored as raw 16-bit words; lo = hi + count)
+  uint16_t *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
   float *X_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
 
   // workspace (per pass of s_batch symmetries)
@@ -88,6 +88,8 @@ void launch_zero(double *p, size_t count, hipStream_t st);
 void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
                      hipStream_t st);
 void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st);
+// fp64 -> bf16 hi/lo planes (dst, dst + count) or fp32
+void launch_convert(const double *src, void *dst, size_t count, int prec, hipStream_t st);
 
 // ---- operator pre/post transforms and harmonic part (operator.hip) ----
 void launch_unpack(const Ensemble &e, const double *psi, int flags, hipStream_t st);

@@ -45,7 +45,24 @@ SIGNATURES = {
         _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "tdscha_apply_full_L": (ctypes.c_int, [
         _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
+    "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "tdscha_profile_reset": (ctypes.c_int, []),
+    "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,
+                                            ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+KGROUP_GEMM1, KGROUP_GEMM2, KGROUP_ROTATE, KGROUP_FAV = 0, 1, 2, 3
+KGROUP_NAMES = {KGROUP_GEMM1: "gemm1", KGROUP_GEMM2: "gemm2", KGROUP_ROTATE: "rotate",
+                KGROUP_FAV: "fav"}
+
+
+def profile_query(group):
+    """(total milliseconds, launches) of a kernel group since the last reset."""
+    ms = ctypes.c_double()
+    cnt = ctypes.c_longlong()
+    check(lib().tdscha_profile_query(int(group), ctypes.byref(ms), ctypes.byref(cnt)),
+          "tdscha_profile_query")
+    return ms.value, cnt.value
 
 _lib = None
 

@@ -68,6 +68,15 @@ def test_random_ensembles(seed, n, N, n_sym, d4):
     _check(ens, R1, Y1, d4)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
+def test_precision_forms(precision, tol):
+    """FP32 / BF16x3 contractions against the FP64 oracle (relative to the abs-sum scale)."""
+    ens = fixtures.random_ensemble(17, 150, 300, 6)
+    R1, Y1 = fixtures.random_perturbation(18, 150)
+    dev = _dev_ens(ens, precisions=("fp64", precision))
+    _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
+
+
 def test_zero_temperature_and_uniform_weights():
     ens = fixtures.random_ensemble(11, 30, 40, 4, T=0.0, weights="uniform")
     R1, Y1 = fixtures.random_perturbation(12, 30)
