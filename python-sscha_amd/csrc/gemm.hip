@@ -286,12 +286,6 @@ constexpr int LB = 128;      // workgroup tile (rows and columns)
 constexpr int LK = 32;       // k slice
 constexpr int LLD = LK + 8;  // LDS row stride in bf16 (80 bytes)
 
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 struct Gemm1BfArgs {
   const uint16_t *Xh, *Xl;    // [Np x npd]

@@ -92,6 +92,132 @@ __global__ void k_rot_left(const double *__restrict__ in, double *__restrict__ o
   }
 }
 
+// ---- Fused two-sided rotations on (row block A) x (column group) tiles --------------
+// A column group is a run of consecutive whole blocks at most kRotTile columns wide, so
+// both one-sided products of a tile stay inside LDS. Used when every block has at most
+// kRotTile modes.
+
+// Ys[s][A][grp] = S_A^T Ysym[A][grp] S_grp   (written as fp64, or as bf16 hi/lo / fp32)
+__global__ __launch_bounds__(256) void k_rot1_fused(const double *__restrict__ Ysym,
+                                                    double *__restrict__ Ys, void *__restrict__ lp,
+                                                    int lp_prec, size_t lp_plane,
+                                                    const double *__restrict__ S,
+                                                    const int4 *__restrict__ blocks,
+                                                    const int2 *__restrict__ groups,
+                                                    const ModeMeta *__restrict__ meta, int npd,
+                                                    int sym_stride, int s0) {
+  __shared__ double M[kRotTile * kRotTile];
+  __shared__ double T[kRotTile * kRotTile];
+  const int4 A = blocks[blockIdx.x];
+  const int2 grp = groups[blockIdx.y];
+  const int sl = blockIdx.z, s = s0 + sl;
+  const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
+  const double *Ss = S + (size_t)s * sym_stride;
+  const double *SA = Ss + A.z;
+  const int cnt = dA * W;
+  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+    const int r = idx / W, c = idx - r * W;
+    M[r * kRotTile + c] = Ysym[(size_t)(offA + r) * npd + g0 + c];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+    const int r = idx / W, c = idx - r * W;
+    const ModeMeta mb = meta[g0 + c];
+    const int cb = mb.blk_off - g0, d = mb.blk_dim, cp = g0 + c - mb.blk_off;
+    const double *SB = Ss + mb.sym_off;
+    double acc = 0.0;
+    for (int l = 0; l < d; ++l) acc = fma(M[r * kRotTile + cb + l], SB[l * d + cp], acc);
+    T[r * kRotTile + c] = acc;
+  }
+  __syncthreads();
+  const size_t base = (size_t)sl * npd * npd;
+  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+    const int r = idx / W, c = idx - r * W;
+    double acc = 0.0;
+    for (int k = 0; k < dA; ++k) acc = fma(SA[k * dA + r], T[k * kRotTile + c], acc);
+    const size_t o = base + (size_t)(offA + r) * npd + g0 + c;
+    if (Ys) Ys[o] = acc;
+    if (lp_prec == kBF16x3) {
+      uint16_t *hi = (uint16_t *)lp;
+      const float f = (float)acc;
+      const uint16_t h = bf16_rne(f);
+      hi[o] = h;
+      hi[lp_plane + o] = bf16_rne(f - __uint_as_float((uint32_t)h << 16));
+    } else if (lp_prec == kFP32) {
+      ((float *)lp)[o] = (float)acc;
+    }
+  }
+}
+
+// d2v[A][grp] += sum_s S_A (B_s + B_s^T)[A][grp] S_grp^T   (one workgroup owns its tile)
+__global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B,
+                                                    double *__restrict__ d2v,
+                                                    const double *__restrict__ S,
+                                                    const int4 *__restrict__ blocks,
+                                                    const int2 *__restrict__ groups,
+                                                    const ModeMeta *__restrict__ meta, int npd,
+                                                    int sym_stride, int s0, int sb) {
+  __shared__ double M[kRotTile * kRotTile];
+  __shared__ double T[kRotTile * kRotTile];
+  constexpr int kPer = kRotTile * kRotTile / 256;
+  const int4 A = blocks[blockIdx.x];
+  const int2 grp = groups[blockIdx.y];
+  const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
+  const int cnt = dA * W;
+  double acc[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) acc[j] = 0.0;
+  for (int sl = 0; sl < sb; ++sl) {
+    const double *Ss = S + (size_t)(s0 + sl) * sym_stride;
+    const double *SA = Ss + A.z;
+    const double *Bs = B + (size_t)sl * npd * npd;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int r = idx / W, c = idx - r * W;
+      M[r * kRotTile + c] = Bs[(size_t)(offA + r) * npd + g0 + c];
+    }
+    // the transposed part, read along rows of B (r fastest) and staged in T
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int c = idx / dA, r = idx - c * dA;
+      T[c * kRotTile + r] = Bs[(size_t)(g0 + c) * npd + offA + r];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int r = idx / W, c = idx - r * W;
+      M[r * kRotTile + c] += T[c * kRotTile + r];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      const int r = idx / W, c = idx - r * W;
+      const ModeMeta mb = meta[g0 + c];
+      const int cb = mb.blk_off - g0, d = mb.blk_dim, cp = g0 + c - mb.blk_off;
+      const double *SB = Ss + mb.sym_off + cp * d;
+      double t = 0.0;
+      for (int l = 0; l < d; ++l) t = fma(M[r * kRotTile + cb + l], SB[l], t);
+      T[r * kRotTile + c] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int idx = threadIdx.x + 256 * j;
+      if (idx < cnt) {
+        const int r = idx / W, c = idx - r * W;
+        double t = 0.0;
+        for (int k = 0; k < dA; ++k) t = fma(SA[r * dA + k], T[k * kRotTile + c], t);
+        acc[j] += t;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int idx = threadIdx.x + 256 * j;
+    if (idx < cnt) {
+      const int r = idx / W, c = idx - r * W;
+      d2v[(size_t)(offA + r) * npd + g0 + c] += acc[j];
+    }
+  }
+}
+
 // r_s = S_s^T R1;  G[s][0][a] = f_ups[a] r_s[a],  G[s][1][a] = r_s[a] / f_psi[a]
 __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G,
                           const double *__restrict__ S, const ModeMeta *__restrict__ meta,
@@ -195,12 +321,6 @@ __global__ void k_finalize(const double *__restrict__ f_acc, const double *__res
   if (do_f && a == 0) f_out[b] = f_acc[b] * inv_norm;
 }
 
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 // fp64 -> bf16 (hi, lo) planes (hi at out, lo at out + count) or fp32, element for element.
 // hi = RNE(v), lo = RNE(v - hi) with v rounded to fp32 first: hi + lo keeps 16 bits.
 __global__ void k_convert(const double *__restrict__ src, void *out, size_t count, int prec) {
@@ -244,6 +364,20 @@ void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, i
   dim3 grid((e.npd + 255) / 256, e.npd, accumulate ? 1 : sb);
   hipLaunchKernelGGL(k_rot_left, grid, dim3(256), 0, st, in, out, e.S, e.meta, e.n, e.npd,
                      e.sym_stride, s0, sb, (int)transS, (int)accumulate);
+}
+
+void launch_rot1_fused(const Ensemble &e, int s0, int sb, int prec, hipStream_t st) {
+  const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+  dim3 grid(e.n_blocks, e.n_groups, sb);
+  hipLaunchKernelGGL(k_rot1_fused, grid, dim3(256), 0, st, e.Ysym,
+                     prec == kFP64 ? e.Ys : (double *)nullptr, e.Ys_lp, prec, plane, e.S,
+                     e.dblocks, e.dgroups, e.meta, e.npd, e.sym_stride, s0);
+}
+
+void launch_rot2_fused(const Ensemble &e, int s0, int sb, hipStream_t st) {
+  dim3 grid(e.n_blocks, e.n_groups);
+  hipLaunchKernelGGL(k_rot2_fused, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dblocks,
+                     e.dgroups, e.meta, e.npd, e.sym_stride, s0, sb);
 }
 
 void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st) {

@@ -170,7 +170,9 @@ int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
     const int sb = (e.N_sym - s0) < e.s_batch ? (e.N_sym - s0) : e.s_batch;
     {
       Span sp(TDSCHA_KGROUP_ROTATE, st);
-      if (need_q) {
+      if (need_q && e.fused_rot) {
+        launch_rot1_fused(e, s0, sb, prec, st);
+      } else if (need_q) {
         launch_rot_right(e, e.Ysym, 0, false, e.Tw, s0, sb, false, st);
         launch_rot_left(e, e.Tw, e.Ys, s0, sb, true, false, st);
         if (prec != kFP64) launch_split_ys(e, sb, prec, st);
@@ -187,8 +189,12 @@ int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
         launch_gemm2(e, sb, prec, st);
       }
       Span sp(TDSCHA_KGROUP_ROTATE, st);
-      launch_rot_right(e, e.B, (long)e.npd * e.npd, true, e.Tw, s0, sb, true, st);
-      launch_rot_left(e, e.Tw, e.d2v_acc, s0, sb, false, true, st);
+      if (e.fused_rot) {
+        launch_rot2_fused(e, s0, sb, st);
+      } else {
+        launch_rot_right(e, e.B, (long)e.npd * e.npd, true, e.Tw, s0, sb, true, st);
+        launch_rot_left(e, e.Tw, e.d2v_acc, s0, sb, false, true, st);
+      }
     }
     if (need_f) {
       Span sp(TDSCHA_KGROUP_FAV, st);
@@ -337,6 +343,20 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= dev_alloc(h, &e.nocc, (size_t)n);
   rc |= dev_alloc(h, &e.S, (size_t)N_sym * e.sym_stride);
   rc |= dev_alloc(h, &e.meta, (size_t)npd);
+  // fused rotations: every block fits a tile; column groups = runs of whole blocks
+  std::vector<int4> hblocks(n_blocks);
+  std::vector<int2> hgroups;
+  e.fused_rot = e.max_block <= kRotTile;
+  for (int b = 0; b < n_blocks; ++b) {
+    hblocks[b] = make_int4(e.blk_off[b], e.blk_dim[b], e.sym_off[b], 0);
+    if (hgroups.empty() || hgroups.back().y + e.blk_dim[b] > kRotTile)
+      hgroups.push_back(make_int2(e.blk_off[b], e.blk_dim[b]));
+    else
+      hgroups.back().y += e.blk_dim[b];
+  }
+  e.n_groups = (int)hgroups.size();
+  rc |= dev_alloc(h, &e.dblocks, (size_t)n_blocks);
+  rc |= dev_alloc(h, &e.dgroups, hgroups.size());
   const size_t sb = e.s_batch, nn = (size_t)npd * npd;
   rc |= dev_alloc(h, &e.Ysym, nn);
   rc |= dev_alloc(h, &e.R1p, (size_t)npd);
@@ -384,6 +404,13 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= up(e.w, w, (size_t)n * 8);
   rc |= up(e.nocc, nocc.data(), nocc.size() * 8);
   rc |= up(e.meta, meta.data(), meta.size() * sizeof(ModeMeta));
+  rc |= up(e.dblocks, hblocks.data(), hblocks.size() * sizeof(int4));
+  rc |= up(e.dgroups, hgroups.data(), hgroups.size() * sizeof(int2));
+  // the fused rotations write only the n x n part of each rotated matrix: the padding
+  // of the workspace must read as zero
+  if (!rc && hipMemset(e.Ys, 0, sb * nn * 8) != hipSuccess) rc = 10;
+  if (!rc && e.Ys_lp && hipMemset(e.Ys_lp, 0, sb * nn * 4) != hipSuccess) rc = 10;
+  if (!rc && hipMemset(e.B, 0, sb * nn * 8) != hipSuccess) rc = 10;
   for (int b = 0; b < n_blocks && !rc; ++b) {
     const size_t d2 = (size_t)e.blk_dim[b] * e.blk_dim[b];
     hipError_t err = hipMemcpy2D(e.S + e.sym_off[b], (size_t)e.sym_stride * 8, symmetries[b], d2 * 8,

@@ -26,7 +26,17 @@ constexpr int kPadRows = 128;  // config padding (GEMM1 row tile, GEMM2 k tile)
 constexpr int kPadCols = 64;   // mode padding (GEMM tiles along modes)
 constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 
+constexpr int kRotTile = 64;   // widest block / column group of the fused rotations
+
 enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
+
+// bf16 as raw 16-bit words: round to nearest even (finite inputs), and back.
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 enum Terms { kTermF = 1, kTermD2vR = 2, kTermD2vY = 4 };
 enum OpFlags {
   kOpTranspose = 1, kOpWigner = 2, kOpIgnoreV3 = 4, kOpIgnoreV4 = 8, kOpIgnoreHarmonic = 16,
@@ -55,6 +65,11 @@ struct Ensemble {
   double *fups = nullptr, *fpsi = nullptr, *S = nullptr;
   double *w = nullptr, *nocc = nullptr;   // frequencies [n] and Bose occupations [n]
   ModeMeta *meta = nullptr;
+  // fused rotations: row blocks (offset, size, sym offset, 0) and column groups (start, width)
+  bool fused_rot = false;
+  int n_groups = 0;
+  int4 *dblocks = nullptr;
+  int2 *dgroups = nullptr;
   // reduced-precision operand copies (bf16 stored as raw 16-bit words; lo = hi + count)
   uint16_t *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
   float *X_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
@@ -82,6 +97,10 @@ void launch_rot_right(const Ensemble &e, const double *in, long in_stride_s, boo
 void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, int sb, bool transS,
                      bool accumulate, hipStream_t st);
 void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st);
+// Ys (fp64 when prec is FP64, else the reduced-precision planes) = S^T Ysym S, fused
+void launch_rot1_fused(const Ensemble &e, int s0, int sb, int prec, hipStream_t st);
+// d2v_acc += sum_s S (B_s + B_s^T) S^T, fused
+void launch_rot2_fused(const Ensemble &e, int s0, int sb, hipStream_t st);
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 void launch_fav_rot_acc(const Ensemble &e, int s0, int sb, hipStream_t st);
 void launch_zero(double *p, size_t count, hipStream_t st);

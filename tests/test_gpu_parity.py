@@ -68,6 +68,17 @@ def test_random_ensembles(seed, n, N, n_sym, d4):
     _check(ens, R1, Y1, d4)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_blocks_wider_than_rotation_tile(precision):
+    """Blocks of more than 64 modes take the unfused rotation kernels."""
+    ens = fixtures.random_ensemble(20, 160, 50, 3, max_block=90)
+    assert max(ens.sym.N_degeneracy) > 64
+    R1, Y1 = fixtures.random_perturbation(24, 160)
+    tol = {"fp64": TOL64, "fp32": 2e-5, "bf16x3": 2e-4}[precision]
+    dev = _dev_ens(ens, precisions=("fp64", precision))
+    _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
 def test_precision_forms(precision, tol):
     """FP32 / BF16x3 contractions against the FP64 oracle (relative to the abs-sum scale)."""
