@@ -93,6 +93,29 @@ def test_full_L_transpose_reproduces_P_recurrence(mode10):
         assert np.linalg.norm(Lp - rhs) / np.linalg.norm(Lp) < 1e-9, k
 
 
+@pytest.mark.parametrize("kw", [dict(), dict(ignore_v4=True), dict(ignore_v3=True),
+                                dict(reverse_L=True, shift_value=2e-6)])
+def test_transposed_L_is_the_exact_transpose(mode10, kw):
+    """<p, L q> = <L^T p, q> for the packed psi basis (what rmatvec relies on, Py:470-479)."""
+    rng = np.random.default_rng(5)
+    n = orc.psi_len(45)
+    p, q = rng.standard_normal(n), rng.standard_normal(n)
+    a = p.dot(orc.apply_full_L(mode10, q, **kw))
+    b = orc.apply_full_L(mode10, p, transpose=True, **kw).dot(q)
+    assert abs(a - b) <= 1e-13 * abs(a)
+
+
+def test_wigner_L_is_self_adjoint_in_masked_product(mode10):
+    """The symmetric Lanczos (run_simm) relies on <p, L q>_mask = <L p, q>_mask."""
+    rng = np.random.default_rng(6)
+    n = orc.psi_len(45)
+    m = orc.mask_dot_wigner(45)
+    p, q = rng.standard_normal(n), rng.standard_normal(n)
+    a = p.dot(m * orc.apply_full_L(mode10, q, use_wigner=True))
+    b = orc.apply_full_L(mode10, p, use_wigner=True).dot(m * q)
+    assert abs(a - b) <= 1e-13 * abs(a)
+
+
 def test_run_FT_kat_and_coefficients(mode10):
     st = fixtures.load_npz("lanczos_std.npz")
     run = orc.run_FT(lambda q: orc.apply_full_L(mode10, q),
