@@ -150,6 +150,16 @@ int tdscha_ensemble_apply(tdscha_ensemble *ens, int precision, int terms, const 
 int tdscha_apply_full_L(tdscha_ensemble *ens, int precision, int flags, double shift,
                         const double *psi, double *out, void *stream);
 
+/* L on n_chains device vectors at once (no reference counterpart: the reference applies L
+ * to one vector per call, Py:6026-6028). Chain c uses flags[c] and shifts[c]; all chains
+ * of a batch share IGNORE_V3 / IGNORE_V4. Every pass over the symmetries serves all chains,
+ * so a biconjugate Lanczos step (L q and L^T p) is one call with two chains. The first call
+ * with more chains than before allocates the per-chain buffers. */
+#define TDSCHA_MAX_CHAINS 64
+int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chains,
+                                const int *flags, const double *shifts,
+                                const double *const *psi, double *const *out, void *stream);
+
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;
  * query synchronises on them and returns the summed elapsed time and the launch count. */

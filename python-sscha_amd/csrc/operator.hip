@@ -127,18 +127,19 @@ __global__ void k_post(const double *__restrict__ psi, double *__restrict__ out,
 
 }  // namespace
 
-void launch_unpack(const Ensemble &e, const double *psi, int flags, hipStream_t st) {
+void launch_unpack(const Ensemble &e, int chain, const double *psi, int flags, hipStream_t st) {
   dim3 grid((e.npd + 255) / 256, e.npd);
-  hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, st, psi, e.Ysym, e.R1p, e.w, e.nocc, e.n, e.npd,
-                     flags);
+  hipLaunchKernelGGL(k_unpack, grid, dim3(256), 0, st, psi, e.Ysym + (size_t)chain * e.npd * e.npd,
+                     e.R1p + (size_t)chain * e.npd, e.w, e.nocc, e.n, e.npd, flags);
 }
 
-void launch_post(const Ensemble &e, const double *psi, double *out, int flags, double shift,
-                 hipStream_t st) {
+void launch_post(const Ensemble &e, int chain, const double *psi, double *out, int flags,
+                 double shift, hipStream_t st) {
   double inv = 1.0 / (e.N_eff * (double)e.N_sym);
   dim3 grid((e.n + 255) / 256, e.n);
-  hipLaunchKernelGGL(k_post, grid, dim3(256), 0, st, psi, out, e.f_acc, e.d2v_acc, inv, e.w, e.nocc,
-                     e.n, e.npd, flags, shift);
+  hipLaunchKernelGGL(k_post, grid, dim3(256), 0, st, psi, out, e.f_acc + (size_t)chain * e.npd,
+                     e.d2v_acc + (size_t)chain * e.npd * e.npd, inv, e.w, e.nocc, e.n, e.npd,
+                     flags, shift);
 }
 
 }  // namespace tdscha

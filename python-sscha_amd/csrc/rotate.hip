@@ -24,12 +24,13 @@ __global__ void k_prep(const double *__restrict__ R1, const double *__restrict__
   if (a == 0) R1p[b] = (R1 && b < n) ? R1[b] : 0.0;
 }
 
-// out[s][k][b] = sum_l M[k][off_B + l] * Sel_B(l, b'),  M = in (+ in^T if symmetrize)
-// Sel_B(l, b') = S_B[l][b'] (transS = 0) or S_B[b'][l] (transS = 1).
-__global__ void k_rot_right(const double *__restrict__ in, long in_stride_s, int symmetrize,
+// out[slot][k][b] = sum_l M[k][off_B + l] * Sel_B(l, b'),  M = in (+ in^T if symmetrize)
+// Sel_B(l, b') = S_B[l][b'] (transS = 0) or S_B[b'][l] (transS = 1). The input matrix is
+// the slot's chain (per_chain = 1) or the slot itself.
+__global__ void k_rot_right(const double *__restrict__ in, int per_chain, int symmetrize,
                             double *__restrict__ out, const double *__restrict__ S,
                             const ModeMeta *__restrict__ meta, int n, int npd, int sym_stride,
-                            int s0, int transS) {
+                            Pass ps, int transS) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y;
   int sl = blockIdx.z;
@@ -37,8 +38,8 @@ __global__ void k_rot_right(const double *__restrict__ in, long in_stride_s, int
   double acc = 0.0;
   if (b < n && k < n) {
     ModeMeta mb = meta[b];
-    const double *Sb = S + (size_t)(s0 + sl) * sym_stride + mb.sym_off;
-    const double *M = in + (size_t)sl * in_stride_s;
+    const double *Sb = S + (size_t)ps.sym(sl) * sym_stride + mb.sym_off;
+    const double *M = in + (size_t)(per_chain ? ps.chain(sl) : sl) * npd * npd;
     const double *row = M + (size_t)k * npd + mb.blk_off;
     const int bp = b - mb.blk_off, d = mb.blk_dim;
     for (int l = 0; l < d; ++l) {
@@ -53,19 +54,23 @@ __global__ void k_rot_right(const double *__restrict__ in, long in_stride_s, int
 
 // out[s][a][b] (or out[a][b] += sum over s) = sum_k Sel_A(a', k) * in[s][off_A + k][b]
 // Sel_A(a', k) = S_A[a'][k] (transS = 0) or S_A[k][a'] (transS = 1).
+// With accumulate, out is per chain (grid z = chains) and sums the chain's slots.
 __global__ void k_rot_left(const double *__restrict__ in, double *__restrict__ out,
                            const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
-                           int npd, int sym_stride, int s0, int sb, int transS, int accumulate) {
+                           int npd, int sym_stride, Pass ps, int transS, int accumulate) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   int a = blockIdx.y;
   if (b >= npd) return;
   if (accumulate) {
     if (a >= n || b >= n) return;
+    const int c = blockIdx.z;
+    out += (size_t)c * npd * npd;
     ModeMeta ma = meta[a];
     const int ap = a - ma.blk_off, d = ma.blk_dim;
     double acc = out[(size_t)a * npd + b];
-    for (int sl = 0; sl < sb; ++sl) {
-      const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+    for (int t = 0; t < ps.ns; ++t) {
+      const int sl = c * ps.ns + t;
+      const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off;
       const double *M = in + (size_t)sl * npd * npd + (size_t)ma.blk_off * npd + b;
       double part = 0.0;
       for (int k = 0; k < d; ++k) {
@@ -81,7 +86,7 @@ __global__ void k_rot_left(const double *__restrict__ in, double *__restrict__ o
     if (a < n && b < n) {
       ModeMeta ma = meta[a];
       const int ap = a - ma.blk_off, d = ma.blk_dim;
-      const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+      const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off;
       const double *M = in + (size_t)sl * npd * npd + (size_t)ma.blk_off * npd + b;
       for (int k = 0; k < d; ++k) {
         double sv = transS ? Sa[k * d + ap] : Sa[ap * d + k];
@@ -105,12 +110,13 @@ __global__ __launch_bounds__(256) void k_rot1_fused(const double *__restrict__ Y
                                                     const int4 *__restrict__ blocks,
                                                     const int2 *__restrict__ groups,
                                                     const ModeMeta *__restrict__ meta, int npd,
-                                                    int sym_stride, int s0) {
+                                                    int sym_stride, Pass ps) {
   __shared__ double M[kRotTile * kRotTile];
   __shared__ double T[kRotTile * kRotTile];
   const int4 A = blocks[blockIdx.x];
   const int2 grp = groups[blockIdx.y];
-  const int sl = blockIdx.z, s = s0 + sl;
+  const int sl = blockIdx.z, s = ps.sym(sl);
+  Ysym += (size_t)ps.chain(sl) * npd * npd;
   const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
   const double *Ss = S + (size_t)s * sym_stride;
   const double *SA = Ss + A.z;
@@ -156,19 +162,22 @@ __global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B
                                                     const int4 *__restrict__ blocks,
                                                     const int2 *__restrict__ groups,
                                                     const ModeMeta *__restrict__ meta, int npd,
-                                                    int sym_stride, int s0, int sb) {
+                                                    int sym_stride, Pass ps) {
   __shared__ double M[kRotTile * kRotTile];
   __shared__ double T[kRotTile * kRotTile];
   constexpr int kPer = kRotTile * kRotTile / 256;
   const int4 A = blocks[blockIdx.x];
   const int2 grp = groups[blockIdx.y];
+  const int ch = blockIdx.z;
+  d2v += (size_t)ch * npd * npd;
   const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
   const int cnt = dA * W;
   double acc[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) acc[j] = 0.0;
-  for (int sl = 0; sl < sb; ++sl) {
-    const double *Ss = S + (size_t)(s0 + sl) * sym_stride;
+  for (int t = 0; t < ps.ns; ++t) {
+    const int sl = ch * ps.ns + t;
+    const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
     const double *SA = Ss + A.z;
     const double *Bs = B + (size_t)sl * npd * npd;
     __syncthreads();
@@ -222,15 +231,16 @@ __global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B
 __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G,
                           const double *__restrict__ S, const ModeMeta *__restrict__ meta,
                           const double *__restrict__ fups, const double *__restrict__ fpsi, int n,
-                          int npd, int sym_stride, int s0) {
+                          int npd, int sym_stride, Pass ps) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   int sl = blockIdx.y;
   if (a >= npd) return;
   double g1 = 0.0, g2 = 0.0;
+  R1p += (size_t)ps.chain(sl) * npd;
   if (a < n) {
     ModeMeta ma = meta[a];
     const int ap = a - ma.blk_off, d = ma.blk_dim;
-    const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off;
+    const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off;
     double r = 0.0;
     for (int k = 0; k < d; ++k) r = fma(Sa[k * d + ap], R1p[ma.blk_off + k], r);
     g1 = fups[a] * r;
@@ -282,17 +292,20 @@ __global__ __launch_bounds__(256) void k_fav(const double *__restrict__ YT,
     }
 }
 
-// f_acc[a] += sum_s sum_k S_s[a'][k] V[s][off_A + k]
+// f_acc[chain][a] += sum over the chain's slots of sum_k S_s[a'][k] V[slot][off_A + k]
 __global__ void k_fav_rot_acc(const double *__restrict__ V, double *__restrict__ f_acc,
                               const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
-                              int npd, int sym_stride, int s0, int sb) {
+                              int npd, int sym_stride, Pass ps) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
   if (a >= n) return;
+  f_acc += (size_t)c * npd;
   ModeMeta ma = meta[a];
   const int ap = a - ma.blk_off, d = ma.blk_dim;
   double acc = f_acc[a];
-  for (int sl = 0; sl < sb; ++sl) {
-    const double *Sa = S + (size_t)(s0 + sl) * sym_stride + ma.sym_off + ap * d;
+  for (int t = 0; t < ps.ns; ++t) {
+    const int sl = c * ps.ns + t;
+    const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off + ap * d;
     const double *v = V + (size_t)sl * npd + ma.blk_off;
     double part = 0.0;
     for (int k = 0; k < d; ++k) part = fma(Sa[k], v[k], part);
@@ -352,38 +365,38 @@ void launch_prep_inputs(const Ensemble &e, const double *R1, const double *Y1, h
   hipLaunchKernelGGL(k_prep, grid, dim3(256), 0, st, R1, Y1, e.Ysym, e.R1p, e.n, e.npd);
 }
 
-void launch_rot_right(const Ensemble &e, const double *in, long in_stride_s, bool symmetrize,
-                      double *out, int s0, int sb, bool transS, hipStream_t st) {
-  dim3 grid((e.npd + 255) / 256, e.npd, sb);
-  hipLaunchKernelGGL(k_rot_right, grid, dim3(256), 0, st, in, in_stride_s, (int)symmetrize, out, e.S,
-                     e.meta, e.n, e.npd, e.sym_stride, s0, (int)transS);
+void launch_rot_right(const Ensemble &e, const double *in, bool per_chain, bool symmetrize,
+                      double *out, const Pass &ps, bool transS, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, e.npd, ps.slots());
+  hipLaunchKernelGGL(k_rot_right, grid, dim3(256), 0, st, in, (int)per_chain, (int)symmetrize, out,
+                     e.S, e.meta, e.n, e.npd, e.sym_stride, ps, (int)transS);
 }
 
-void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, int sb, bool transS,
+void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pass &ps, bool transS,
                      bool accumulate, hipStream_t st) {
-  dim3 grid((e.npd + 255) / 256, e.npd, accumulate ? 1 : sb);
+  dim3 grid((e.npd + 255) / 256, e.npd, accumulate ? ps.nc : ps.slots());
   hipLaunchKernelGGL(k_rot_left, grid, dim3(256), 0, st, in, out, e.S, e.meta, e.n, e.npd,
-                     e.sym_stride, s0, sb, (int)transS, (int)accumulate);
+                     e.sym_stride, ps, (int)transS, (int)accumulate);
 }
 
-void launch_rot1_fused(const Ensemble &e, int s0, int sb, int prec, hipStream_t st) {
+void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st) {
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
-  dim3 grid(e.n_blocks, e.n_groups, sb);
+  dim3 grid(e.n_blocks, e.n_groups, ps.slots());
   hipLaunchKernelGGL(k_rot1_fused, grid, dim3(256), 0, st, e.Ysym,
                      prec == kFP64 ? e.Ys : (double *)nullptr, e.Ys_lp, prec, plane, e.S,
-                     e.dblocks, e.dgroups, e.meta, e.npd, e.sym_stride, s0);
+                     e.dblocks, e.dgroups, e.meta, e.npd, e.sym_stride, ps);
 }
 
-void launch_rot2_fused(const Ensemble &e, int s0, int sb, hipStream_t st) {
-  dim3 grid(e.n_blocks, e.n_groups);
+void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st) {
+  dim3 grid(e.n_blocks, e.n_groups, ps.nc);
   hipLaunchKernelGGL(k_rot2_fused, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dblocks,
-                     e.dgroups, e.meta, e.npd, e.sym_stride, s0, sb);
+                     e.dgroups, e.meta, e.npd, e.sym_stride, ps);
 }
 
-void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st) {
-  dim3 grid((e.npd + 255) / 256, sb);
+void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {
+  dim3 grid((e.npd + 255) / 256, ps.slots());
   hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, e.R1p, e.G, e.S, e.meta, e.fups, e.fpsi,
-                     e.n, e.npd, e.sym_stride, s0);
+                     e.n, e.npd, e.sym_stride, ps);
 }
 
 void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
@@ -394,9 +407,9 @@ void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
   }
 }
 
-void launch_fav_rot_acc(const Ensemble &e, int s0, int sb, hipStream_t st) {
-  hipLaunchKernelGGL(k_fav_rot_acc, dim3((e.n + 255) / 256), dim3(256), 0, st, e.V, e.f_acc, e.S,
-                     e.meta, e.n, e.npd, e.sym_stride, s0, sb);
+void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st) {
+  hipLaunchKernelGGL(k_fav_rot_acc, dim3((e.n + 255) / 256, ps.nc), dim3(256), 0, st, e.V, e.f_acc,
+                     e.S, e.meta, e.n, e.npd, e.sym_stride, ps);
 }
 
 void launch_zero(double *p, size_t count, hipStream_t st) {

@@ -156,28 +156,31 @@ struct Span {
   }
 };
 
-// Per-pass orchestration of one ensemble application. Y1/R1 must already be staged in
-// e.Ysym / e.R1p (prep or unpack kernel). Leaves unnormalised sums in f_acc / d2v_acc.
-int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
+// Per-pass orchestration of one ensemble application for nc chains. Y1/R1 of chain c must
+// already be staged in e.Ysym / e.R1p + c (prep or unpack kernel). Leaves unnormalised sums
+// in f_acc / d2v_acc of every chain.
+int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st) {
   const bool need_q = what & (kTermF | kTermD2vY);
   const bool need_r = what & kTermD2vR;
   const bool need_d = what & (kTermD2vR | kTermD2vY);
   const bool need_f = what & kTermF;
-  launch_zero(e.f_acc, e.npd, st);
-  launch_zero(e.d2v_acc, (size_t)e.npd * e.npd, st);
+  launch_zero(e.f_acc, (size_t)nc * e.npd, st);
+  launch_zero(e.d2v_acc, (size_t)nc * e.npd * e.npd, st);
   if (!need_r) launch_zero(e.G, (size_t)e.s_batch * 2 * e.npd, st);
-  for (int s0 = 0; s0 < e.N_sym; s0 += e.s_batch) {
-    const int sb = (e.N_sym - s0) < e.s_batch ? (e.N_sym - s0) : e.s_batch;
+  const int ns_max = e.s_batch / nc > 0 ? e.s_batch / nc : 1;
+  for (int s0 = 0; s0 < e.N_sym; s0 += ns_max) {
+    const Pass ps{s0, (e.N_sym - s0) < ns_max ? (e.N_sym - s0) : ns_max, nc};
+    const int sb = ps.slots();
     {
       Span sp(TDSCHA_KGROUP_ROTATE, st);
       if (need_q && e.fused_rot) {
-        launch_rot1_fused(e, s0, sb, prec, st);
+        launch_rot1_fused(e, ps, prec, st);
       } else if (need_q) {
-        launch_rot_right(e, e.Ysym, 0, false, e.Tw, s0, sb, false, st);
-        launch_rot_left(e, e.Tw, e.Ys, s0, sb, true, false, st);
+        launch_rot_right(e, e.Ysym, true, false, e.Tw, ps, false, st);
+        launch_rot_left(e, e.Tw, e.Ys, ps, true, false, st);
         if (prec != kFP64) launch_split_ys(e, sb, prec, st);
       }
-      if (need_r) launch_rot_vec(e, s0, sb, st);
+      if (need_r) launch_rot_vec(e, ps, st);
     }
     {
       Span sp(TDSCHA_KGROUP_GEMM1, st);
@@ -190,16 +193,16 @@ int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
       }
       Span sp(TDSCHA_KGROUP_ROTATE, st);
       if (e.fused_rot) {
-        launch_rot2_fused(e, s0, sb, st);
+        launch_rot2_fused(e, ps, st);
       } else {
-        launch_rot_right(e, e.B, (long)e.npd * e.npd, true, e.Tw, s0, sb, true, st);
-        launch_rot_left(e, e.Tw, e.d2v_acc, s0, sb, false, true, st);
+        launch_rot_right(e, e.B, false, true, e.Tw, ps, true, st);
+        launch_rot_left(e, e.Tw, e.d2v_acc, ps, false, true, st);
       }
     }
     if (need_f) {
       Span sp(TDSCHA_KGROUP_FAV, st);
       launch_fav(e, sb, st);
-      launch_fav_rot_acc(e, s0, sb, st);
+      launch_fav_rot_acc(e, ps, st);
     }
   }
   hipError_t err = hipGetLastError();
@@ -207,6 +210,57 @@ int run_passes(const Ensemble &e, int prec, int what, hipStream_t st) {
     set_error(std::string("kernel launch failed: ") + hipGetErrorString(err));
     return 12;
   }
+  return 0;
+}
+
+// Allocate the per-pass workspace for `slots` (chain, symmetry) slots. The padding of the
+// rotated matrices and of B must read as zero (the fused rotations write only n x n).
+int alloc_slots(tdscha_ensemble *h, int slots) {
+  Ensemble &e = h->e;
+  const size_t sb = (size_t)slots, nn = (size_t)e.npd * e.npd;
+  int rc = 0;
+  rc |= dev_alloc(h, &e.Tw, sb * nn);
+  rc |= dev_alloc(h, &e.Ys, sb * nn);
+  rc |= dev_alloc(h, &e.G, sb * 2 * e.npd);
+  rc |= dev_alloc(h, &e.W, sb * 4 * e.Np);
+  rc |= dev_alloc(h, &e.B, sb * nn);
+  rc |= dev_alloc(h, &e.V, sb * e.npd);
+  if (e.prec_mask & ((1 << kFP32) | (1 << kBF16x3))) {
+    char *lp = nullptr;
+    rc |= dev_alloc(h, &lp, sb * nn * 4);  // fp32 plane or bf16 hi + lo planes
+    e.Ys_lp = lp;
+  }
+  if (rc) return rc;
+  if (hipMemset(e.Ys, 0, sb * nn * 8) != hipSuccess ||
+      (e.Ys_lp && hipMemset(e.Ys_lp, 0, sb * nn * 4) != hipSuccess) ||
+      hipMemset(e.B, 0, sb * nn * 8) != hipSuccess) {
+    set_error("workspace initialisation failed");
+    return 10;
+  }
+  e.s_batch = slots;
+  return 0;
+}
+
+// Grow the per-chain buffers to hold nc chains (allocates on the first call with more),
+// and the slot workspace when a pass cannot hold one symmetry of every chain.
+int ensure_chains(tdscha_ensemble *h, int nc) {
+  Ensemble &e = h->e;
+  if (nc > e.s_batch) {
+    int rc = alloc_slots(h, nc);
+    if (rc) return rc;
+  }
+  if (nc <= e.n_chains) return 0;
+  const size_t nn = (size_t)e.npd * e.npd;
+  double *ys = nullptr, *r1 = nullptr, *dv = nullptr, *fa = nullptr;
+  int rc = 0;
+  rc |= dev_alloc(h, &ys, (size_t)nc * nn);
+  rc |= dev_alloc(h, &r1, (size_t)nc * e.npd);
+  rc |= dev_alloc(h, &dv, (size_t)nc * nn);
+  rc |= dev_alloc(h, &fa, (size_t)nc * e.npd);
+  if (rc) return rc;
+  // superseded buffers stay in the allocation list and are released with the ensemble
+  e.Ysym = ys; e.R1p = r1; e.d2v_acc = dv; e.f_acc = fa;
+  e.n_chains = nc;
   return 0;
 }
 
@@ -357,25 +411,13 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   e.n_groups = (int)hgroups.size();
   rc |= dev_alloc(h, &e.dblocks, (size_t)n_blocks);
   rc |= dev_alloc(h, &e.dgroups, hgroups.size());
-  const size_t sb = e.s_batch, nn = (size_t)npd * npd;
-  rc |= dev_alloc(h, &e.Ysym, nn);
-  rc |= dev_alloc(h, &e.R1p, (size_t)npd);
-  rc |= dev_alloc(h, &e.Tw, sb * nn);
-  rc |= dev_alloc(h, &e.Ys, sb * nn);
-  rc |= dev_alloc(h, &e.G, sb * 2 * npd);
-  rc |= dev_alloc(h, &e.W, sb * 4 * Np);
-  rc |= dev_alloc(h, &e.B, sb * nn);
-  rc |= dev_alloc(h, &e.V, sb * npd);
-  rc |= dev_alloc(h, &e.d2v_acc, nn);
-  rc |= dev_alloc(h, &e.f_acc, (size_t)npd);
+  if (!rc) rc |= alloc_slots(h, e.s_batch);
+  if (!rc) rc |= ensure_chains(h, 1);
   const bool want32 = e.prec_mask & (1 << kFP32), wantbf = e.prec_mask & (1 << kBF16x3);
   const size_t NX = (size_t)Np * npd;
   if (want32 || wantbf) {
     rc |= dev_alloc(h, &e.XtT_f, NX);
     rc |= dev_alloc(h, &e.YT_f, NX);
-    char *lp = nullptr;
-    rc |= dev_alloc(h, &lp, sb * nn * 4);  // fp32 plane or bf16 hi + lo planes
-    e.Ys_lp = lp;
   }
   if (want32) rc |= dev_alloc(h, &e.X_f, NX);
   if (wantbf) {
@@ -406,11 +448,6 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= up(e.meta, meta.data(), meta.size() * sizeof(ModeMeta));
   rc |= up(e.dblocks, hblocks.data(), hblocks.size() * sizeof(int4));
   rc |= up(e.dgroups, hgroups.data(), hgroups.size() * sizeof(int2));
-  // the fused rotations write only the n x n part of each rotated matrix: the padding
-  // of the workspace must read as zero
-  if (!rc && hipMemset(e.Ys, 0, sb * nn * 8) != hipSuccess) rc = 10;
-  if (!rc && e.Ys_lp && hipMemset(e.Ys_lp, 0, sb * nn * 4) != hipSuccess) rc = 10;
-  if (!rc && hipMemset(e.B, 0, sb * nn * 8) != hipSuccess) rc = 10;
   for (int b = 0; b < n_blocks && !rc; ++b) {
     const size_t d2 = (size_t)e.blk_dim[b] * e.blk_dim[b];
     hipError_t err = hipMemcpy2D(e.S + e.sym_off[b], (size_t)e.sym_stride * 8, symmetries[b], d2 * 8,
@@ -475,7 +512,7 @@ int tdscha_ensemble_apply(tdscha_ensemble *h, int precision, int terms, const do
   hipStream_t st = (hipStream_t)stream;
   // Stage inputs; a NULL operand is staged as zeros.
   launch_prep_inputs(e, R1, Y1, st);
-  rc = run_passes(e, precision, terms, st);
+  rc = run_passes(e, precision, terms, 1, st);
   if (rc) return rc;
   launch_finalize(e, terms, d2v_accumulate != 0, f, d2v, st);
   TD_CHECK(hipGetLastError());
@@ -484,20 +521,40 @@ int tdscha_ensemble_apply(tdscha_ensemble *h, int precision, int terms, const do
 
 int tdscha_apply_full_L(tdscha_ensemble *h, int precision, int flags, double shift,
                         const double *psi, double *out, void *stream) {
-  if (!h || !psi || !out) { set_error("NULL argument"); return 1; }
-  if (psi == out) { set_error("out must not alias psi"); return 1; }
+  return tdscha_apply_full_L_batched(h, precision, 1, &flags, &shift, &psi, &out, stream);
+}
+
+int tdscha_apply_full_L_batched(tdscha_ensemble *h, int precision, int n_chains, const int *flags,
+                                const double *shifts, const double *const *psi,
+                                double *const *out, void *stream) {
+  if (!h || !flags || !shifts || !psi || !out) { set_error("NULL argument"); return 1; }
+  if (n_chains < 1 || n_chains > TDSCHA_MAX_CHAINS) {
+    set_error("n_chains must be in [1, " + std::to_string(TDSCHA_MAX_CHAINS) + "]");
+    return 1;
+  }
+  const int anh_mask = kOpIgnoreV3 | kOpIgnoreV4;
+  for (int c = 0; c < n_chains; ++c) {
+    if (!psi[c] || !out[c]) { set_error("NULL psi / out"); return 1; }
+    if (psi[c] == out[c]) { set_error("out must not alias psi"); return 1; }
+    if ((flags[c] & anh_mask) != (flags[0] & anh_mask)) {
+      set_error("IGNORE_V3 / IGNORE_V4 must be the same for every chain of a batch");
+      return 1;
+    }
+  }
   Ensemble &e = h->e;
   int rc = check_prec(e, precision);
   if (rc) return rc;
   TD_CHECK(hipSetDevice(e.device));
+  rc = ensure_chains(h, n_chains);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  if (!(flags & kOpIgnoreV3)) {
-    launch_unpack(e, psi, flags, st);
-    int what = kTermF | kTermD2vR | ((flags & kOpIgnoreV4) ? 0 : kTermD2vY);
-    rc = run_passes(e, precision, what, st);
+  if (!(flags[0] & kOpIgnoreV3)) {
+    for (int c = 0; c < n_chains; ++c) launch_unpack(e, c, psi[c], flags[c], st);
+    int what = kTermF | kTermD2vR | ((flags[0] & kOpIgnoreV4) ? 0 : kTermD2vY);
+    rc = run_passes(e, precision, what, n_chains, st);
     if (rc) return rc;
   }
-  launch_post(e, psi, out, flags, shift, st);
+  for (int c = 0; c < n_chains; ++c) launch_post(e, c, psi[c], out[c], flags[c], shifts[c], st);
   TD_CHECK(hipGetLastError());
   return 0;
 }

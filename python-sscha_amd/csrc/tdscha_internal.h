@@ -43,6 +43,15 @@ enum OpFlags {
   kOpReverse = 32
 };
 
+// One pass of the ensemble loop: nc chains x ns symmetries [s0, s0 + ns). Workspace slot j
+// holds chain j / ns and symmetry s0 + j % ns.
+struct Pass {
+  int s0, ns, nc;
+  __host__ __device__ int slots() const { return ns * nc; }
+  __host__ __device__ int chain(int j) const { return j / ns; }
+  __host__ __device__ int sym(int j) const { return s0 + j % ns; }
+};
+
 // Per-mode symmetry metadata on the device.
 struct ModeMeta {
   int blk_off;   // first mode of the block this mode belongs to
@@ -74,9 +83,13 @@ struct Ensemble {
   uint16_t *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
   float *X_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
 
-  // workspace (per pass of s_batch symmetries)
-  double *Ysym = nullptr;     // [npd x npd]  (Y1 + Y1^T)/2, padded
-  double *R1p = nullptr;      // [npd]
+  // per-chain inputs / accumulators, for up to n_chains chains (grown on demand)
+  int n_chains = 0;
+  double *Ysym = nullptr;     // [chains x npd x npd]  (Y1 + Y1^T)/2, padded
+  double *R1p = nullptr;      // [chains x npd]
+  double *d2v_acc = nullptr;  // [chains x npd x npd]
+  double *f_acc = nullptr;    // [chains x npd]
+  // workspace (per pass of s_batch slots)
   double *Tw = nullptr;       // [s_batch x npd x npd]  rotation intermediate
   double *Ys = nullptr;       // [s_batch x npd x npd]  S_s^T Ysym S_s
   void *Ys_lp = nullptr;      // reduced-precision Ys (bf16 hi+lo or fp32)
@@ -84,25 +97,23 @@ struct Ensemble {
   double *W = nullptr;        // [s_batch x 4 x Np]  c1, c2, e1, e2 per image
   double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
-  double *d2v_acc = nullptr;  // [npd x npd]
-  double *f_acc = nullptr;    // [npd]
 };
 
 void set_error(const std::string &msg);
 
 // ---- kernels / launchers (rotate.hip) ----
 void launch_prep_inputs(const Ensemble &e, const double *R1, const double *Y1, hipStream_t st);
-void launch_rot_right(const Ensemble &e, const double *in, long in_stride_s, bool symmetrize,
-                      double *out, int s0, int sb, bool transS, hipStream_t st);
-void launch_rot_left(const Ensemble &e, const double *in, double *out, int s0, int sb, bool transS,
+void launch_rot_right(const Ensemble &e, const double *in, bool per_chain, bool symmetrize,
+                      double *out, const Pass &ps, bool transS, hipStream_t st);
+void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pass &ps, bool transS,
                      bool accumulate, hipStream_t st);
-void launch_rot_vec(const Ensemble &e, int s0, int sb, hipStream_t st);
+void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st);
 // Ys (fp64 when prec is FP64, else the reduced-precision planes) = S^T Ysym S, fused
-void launch_rot1_fused(const Ensemble &e, int s0, int sb, int prec, hipStream_t st);
-// d2v_acc += sum_s S (B_s + B_s^T) S^T, fused
-void launch_rot2_fused(const Ensemble &e, int s0, int sb, hipStream_t st);
+void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st);
+// d2v_acc[chain] += sum over the chain's slots of S (B_s + B_s^T) S^T, fused
+void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st);
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
-void launch_fav_rot_acc(const Ensemble &e, int s0, int sb, hipStream_t st);
+void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st);
 void launch_zero(double *p, size_t count, hipStream_t st);
 void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
                      hipStream_t st);
@@ -111,9 +122,9 @@ void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st);
 void launch_convert(const double *src, void *dst, size_t count, int prec, hipStream_t st);
 
 // ---- operator pre/post transforms and harmonic part (operator.hip) ----
-void launch_unpack(const Ensemble &e, const double *psi, int flags, hipStream_t st);
-void launch_post(const Ensemble &e, const double *psi, double *out, int flags, double shift,
-                 hipStream_t st);
+void launch_unpack(const Ensemble &e, int chain, const double *psi, int flags, hipStream_t st);
+void launch_post(const Ensemble &e, int chain, const double *psi, double *out, int flags,
+                 double shift, hipStream_t st);
 
 // ---- GEMMs (gemm.hip) ----
 // GEMM1: per image (config i, symmetry s) the quadratic forms

@@ -333,11 +333,16 @@ class Lanczos(object):
                 print("\n ===== NEW STEP %d =====\n" % (i + 1))
                 if flush_output:
                     sys.stdout.flush()
-            L_q = self.apply_full_L_device(psi_q, transpose=False, out=work_q).clone()
             if run_simm:
+                L_q = self.apply_full_L_device(psi_q, transpose=False, out=work_q).clone()
                 p_L = L_q.clone()
             else:
-                p_L = self.apply_full_L_device(psi_p, transpose=True, out=work_p).clone()
+                # L q and L^T p share one pass over the symmetries
+                L_q, p_L = ens.apply_full_L_batched(
+                    [psi_q, psi_p], flags=[self._flags(False), self._flags(True)],
+                    shifts=[float(self.shift_value)] * 2, precision=self.precision,
+                    outs=[work_q, work_p])
+                L_q, p_L = L_q.clone(), p_L.clone()
             c_old = c_l[-1] if len(c_l) > 0 else 1.0
             p_norm = s_norm_l[-1] / c_old
             a_coeff = float(dot(psi_p, L_q)) * p_norm

@@ -45,6 +45,8 @@ SIGNATURES = {
         _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]),
     "tdscha_apply_full_L": (ctypes.c_int, [
         _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
+    "tdscha_apply_full_L_batched": (ctypes.c_int, [
+        _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

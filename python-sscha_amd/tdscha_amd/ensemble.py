@@ -152,3 +152,23 @@ class DeviceEnsemble:
                                          _stream_handle(t, self.device))
         L.check(rc, "tdscha_apply_full_L")
         return out
+
+    def apply_full_L_batched(self, psis, flags=None, shifts=None, precision="fp64", outs=None):
+        """L on several device vectors in one pass over the symmetries
+        (tdscha_apply_full_L_batched). flags / shifts per vector."""
+        t = self.torch
+        n = self.n
+        C = len(psis)
+        psis = [self._dev(p, (n + n * (n + 1),)) for p in psis]
+        if outs is None:
+            outs = [t.empty_like(p) for p in psis]
+        flags = [0] * C if flags is None else list(flags)
+        shifts = [0.0] * C if shifts is None else list(shifts)
+        fl = (ctypes.c_int * C)(*[int(f) for f in flags])
+        sh = (ctypes.c_double * C)(*[float(s) for s in shifts])
+        pp = (ctypes.c_void_p * C)(*[p.data_ptr() for p in psis])
+        po = (ctypes.c_void_p * C)(*[o.data_ptr() for o in outs])
+        rc = L.lib().tdscha_apply_full_L_batched(self._h, L.PRECISIONS[precision], C, fl, sh, pp, po,
+                                                 _stream_handle(t, self.device))
+        L.check(rc, "tdscha_apply_full_L_batched")
+        return outs

@@ -227,6 +227,32 @@ def test_full_L_wigner_and_flags(mode10):
             setattr(lz, key, type(getattr(lz, key))(0))
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("bf16x3", 1e-12)])
+def test_batched_chains_equal_single_applications(mode10, precision, tol):
+    """Several chains with different flags in one pass give what separate calls give."""
+    import torch
+    from tdscha_amd import _lib as L
+    ens = fixtures.random_ensemble(33, 60, 90, 7)
+    dev = _dev_ens(ens, precisions=("fp64", precision), s_batch=3)   # fewer slots than chains
+    rng = np.random.default_rng(34)
+    n_psi = orc.psi_len(60)
+    psis = [torch.as_tensor(rng.standard_normal(n_psi), device=dev.device) for _ in range(5)]
+    flags = [0, L.OP_TRANSPOSE, L.OP_WIGNER, L.OP_REVERSE, L.OP_TRANSPOSE | L.OP_REVERSE]
+    shifts = [0.0, 1e-6, 0.0, 2e-6, 0.0]
+    outs = dev.apply_full_L_batched(psis, flags, shifts, precision=precision)
+    for p, f, s, o in zip(psis, flags, shifts, outs):
+        ref = dev.apply_full_L(p, flags=f, shift=s, precision=precision)
+        err = (torch.linalg.norm(o - ref) / torch.linalg.norm(ref)).item()
+        assert err < tol, (f, err)
+    # and against the oracle for the fp64 chains
+    if precision == "fp64":
+        for p, f, s, o in zip(psis, flags, shifts, outs):
+            kw = dict(transpose=bool(f & L.OP_TRANSPOSE), use_wigner=bool(f & L.OP_WIGNER),
+                      reverse_L=bool(f & L.OP_REVERSE), shift_value=s)
+            ref = orc.apply_full_L(ens, p.cpu().numpy(), **kw)
+            assert np.linalg.norm(o.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-11
+
+
 def test_run_FT_coefficients_and_kat(mode10):
     """a/b/c of the reference's 10-step run and the KAT c_4 = 6.59744344e-07
     (tests/test_julia/test_lanczos.py:60-62)."""
