@@ -40,6 +40,13 @@ __device__ inline v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Row of result element r held by lane group lg = lane / 16 (the column is lane % 16).
+// Measured on gfx950: the f32 (and bf16) 16x16 results put rows 4 lg .. 4 lg + 3 in one
+// lane group; the f64 16x16x4 result interleaves them, row lg + 4 r.
+template <class T> __device__ inline int out_row(int lg, int r);
+template <> __device__ inline int out_row<float>(int lg, int r) { return 4 * lg + r; }
+template <> __device__ inline int out_row<double>(int lg, int r) { return lg + 4 * r; }
+
 // ---------------------------------------------------------------- FP64 / FP32 (16x16x4)
 constexpr int BM = 64, BN = 64, BK = 16;
 constexpr int LDS_LD = BM + 16;  // [k][m] row stride: consecutive k rows land 16 or 32 banks apart
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(256) void k_gemm1(Gemm1Args<T> g) {
       for (int ti = 0; ti < 2; ++ti) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = wr * 32 + ti * 16 + 4 * lg + r;
+          const int m = wr * 32 + ti * 16 + out_row<T>(lg, r);
           const size_t o = (size_t)(i0 + m) * g.npd + c;
           const double x = g.X[o], ft = g.Ft[o];
           const double q = (double)acc[ti][tj][r];
@@ -197,7 +204,7 @@ __global__ __launch_bounds__(256) void k_gemm1(Gemm1Args<T> g) {
         pd[ti][r] += __shfl_xor(pd[ti][r], off, 64);
       }
       if (lr == 0) {
-        const int m = wr * 32 + ti * 16 + 4 * lg + r;
+        const int m = wr * 32 + ti * 16 + out_row<T>(lg, r);
         red[wc][m][0] = pa[ti][r];
         red[wc][m][1] = pb[ti][r];
         red[wc][m][2] = pc[ti][r];
@@ -275,7 +282,7 @@ __global__ __launch_bounds__(256) void k_gemm2(Gemm2Args<T> g) {
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wr * 32 + ti * 16 + 4 * lg + r;
+        const int m = m0 + wr * 32 + ti * 16 + out_row<T>(lg, r);
         const int c = c0 + wc * 32 + tj * 16 + lr;
         Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
       }
