@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=sorted(PEAK_TFLOPS))
     ap.add_argument("--s-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64-sample-syms", type=int, default=-1,
+                    help="symmetries of the FP64 comparison sample (-1: N_sym/32, 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -145,6 +147,32 @@ def main():
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                 "kernel": "k_%s_%s" % (dom, "f64" if args.precision == "fp64" else args.precision)}
     value = args.steps / elapsed
+    fp64_ref = None
+    if args.precision != "fp64" and args.fp64_sample_syms != 0:
+        # the plain FP64 path on the first k symmetries: its cost is linear in the number of
+        # symmetry passes, so the full-ensemble rate is the sample rate times k / N_sym
+        k = args.fp64_sample_syms if args.fp64_sample_syms > 0 else max(1, ens.n_syms // 32)
+        k = min(k, ens.n_syms)
+        sub = SymmetryBlocks([s[:k] for s in ens.symmetries], ens.N_degeneracy,
+                             ens.degenerate_space, ens.block_id)
+        del dens
+        torch.cuda.empty_cache()
+        d64 = DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sub, s_batch=args.s_batch)
+        d64.apply_full_L(psi, precision="fp64", out=out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d64.apply_full_L(psi, precision="fp64", out=out)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        v64 = 1.0 / (t64 * ens.n_syms / k)
+        fp64_ref = {"value": v64, "unit": "L-applications/s",
+                    "sample": "plain FP64 path, one L application over the first %d of %d "
+                              "symmetries (%.3f s), scaled by %d/%d" % (k, ens.n_syms, t64,
+                                                                      ens.n_syms, k),
+                    "speedup": value / v64}
+        log("fp64 plain: %.4g L/s (sample %d syms, %.3f s) -> speedup %.2fx" % (
+            v64, k, t64, value / v64))
+        d64.close()
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(ens)
@@ -169,6 +197,7 @@ def main():
                    "achieved_TFLOPs_whole_L": round(flop_l * value / 1e12, 3)},
         "roofline": roofline,
         "cpu_baseline": base,
+        "fp64_plain": fp64_ref,
     }
     print(json.dumps(res), flush=True)
 
