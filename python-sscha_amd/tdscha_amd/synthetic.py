@@ -289,7 +289,7 @@ def symmetry_blocks(cr: Crystal, E, blocks, chunk=256, device=None):
 
 
 def make_ensemble(L, n_configs, T=300.0, weights="uniform", seed_fc=0, seed_cfg=1, seed_w=2,
-                  device=None, chunk_configs=2000):
+                  device=None, chunk_configs=2000, sigma3=0.1, sigma4=0.05):
     """The synthetic ensemble of an L x L x L supercell (see the module docstring)."""
     cr = make_crystal(L)
     phi = force_constants(cr)
@@ -300,7 +300,7 @@ def make_ensemble(L, n_configs, T=300.0, weights="uniform", seed_fc=0, seed_cfg=
     import torch
     dev = device if device is not None else torch.device("cpu")
     rng = np.random.default_rng(seed_cfg)
-    tens = bond_tensors(seed_fc)
+    tens = bond_tensors(seed_fc, sigma3, sigma4)
     m3 = np.repeat(cr.masses, 3)
     Et = torch.as_tensor(E, device=dev)
     sm = torch.as_tensor(np.sqrt(m3), device=dev)

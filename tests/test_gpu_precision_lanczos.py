@@ -1,7 +1,10 @@
 """Lanczos with the reduced-precision contractions against the FP64 device path.
 
 Workload: synthetic rock-salt 2x2x2 supercell (16 atoms, 45 modes), 2000 configs, the full
-384-image symmetry set, T = 300 K, IR perturbation along x with Z* = +-2. Criteria: Lanczos
+384-image symmetry set, T = 300 K, IR perturbation along x with Z* = +-2. The bond
+anharmonicity is 20x the benchmark model's (sigma 2 eV/A^3 and 1 eV/A^4): with the weaker
+model the first Lanczos residual (3e-13) is below the reference's absolute convergence
+threshold of 1e-12 (Py:121, Py:6144-6153) and the recurrence stops at once. Criteria: Lanczos
 a / b / c within 1e-5 relative and the peak positions of -Im G(w) (0-300 cm^-1, 1 cm^-1
 smearing) within 0.1 cm^-1 of FP64, each beside the ensemble's own stochastic spread
 (the same Lanczos on two disjoint halves of the configurations).
@@ -18,7 +21,7 @@ N_STEPS = 50
 def rocksalt2():
     import torch
     from tdscha_amd import synthetic as sy
-    return sy.make_ensemble(2, 2000, T=300.0, device=torch.device("cuda"))
+    return sy.make_ensemble(2, 2000, T=300.0, device=torch.device("cuda"), sigma3=2.0, sigma4=1.0)
 
 
 def _run(ens, mode, cfg=slice(None), steps=N_STEPS, full_gs=True):
@@ -57,6 +60,7 @@ def test_reduced_precision_lanczos(rocksalt2, mode_name):
     from tdscha_amd import DynamicalLanczos as DL
     ref = _run(rocksalt2, DL.MODE_GPU_FP64)
     lp = _run(rocksalt2, getattr(DL, mode_name))
+    assert len(ref.b_coeffs) >= 10, "the FP64 Lanczos stopped after %d steps" % len(ref.b_coeffs)
     a0, b0, c0 = map(np.asarray, (ref.a_coeffs, ref.b_coeffs, ref.c_coeffs))
     a1, b1, c1 = map(np.asarray, (lp.a_coeffs, lp.b_coeffs, lp.c_coeffs))
     k = min(len(a0), len(a1))
