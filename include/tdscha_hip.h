@@ -160,6 +160,33 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
                                 const int *flags, const double *shifts,
                                 const double *const *psi, double *const *out, void *stream);
 
+/* Allocate the per-chain buffers for n_chains chains now (what the first batched call with
+ * that many chains would do), so that later calls allocate nothing. */
+int tdscha_ensemble_reserve(tdscha_ensemble *ens, int n_chains);
+
+/* ---- Device-resident Lanczos (the recurrence of Lanczos.run_FT, Py:5843-6372) ---------
+ * n_chains starting vectors advance together: every step is one batched L application
+ * (L q and L^T p of every chain, or L q only for the symmetric Wigner form) followed by the
+ * FP64 recurrence on the device; a step is captured once as a hipGraph and replayed. There
+ * is no re-orthogonalisation (the reference's default, n_rep_orth = 0). flags / shifts per
+ * chain as for tdscha_apply_full_L (the TRANSPOSE bit is managed internally). mask (host,
+ * length n + n (n + 1)) is the Wigner dot-product weight, needed when symmetric is 1.
+ * keep_basis stores every q (and p) on the device. */
+typedef struct tdscha_lanczos tdscha_lanczos;
+int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precision,
+                          int n_chains, const int *flags, const double *shifts, int symmetric,
+                          int max_steps, int keep_basis, const double *mask);
+int tdscha_lanczos_destroy(tdscha_lanczos *lz);
+/* psi0[c]: device vectors; they are normalised as in run_FT (Py:5936-5945). */
+int tdscha_lanczos_start(tdscha_lanczos *lz, const double *const *psi0, void *stream);
+int tdscha_lanczos_run(tdscha_lanczos *lz, int n_steps, int use_graph, void *stream);
+/* Host copies of the first steps_done a / b / c / s_norm_(k+1) of a chain (any may be NULL);
+ * synchronises the device. */
+int tdscha_lanczos_read(tdscha_lanczos *lz, int chain, double *a, double *b, double *c,
+                        double *s_norm, int *steps_done);
+/* Device pointers to the stored basis of a chain ((max_steps + 1) x (n + n (n + 1))). */
+int tdscha_lanczos_basis(tdscha_lanczos *lz, int chain, const double **Q, const double **P);
+
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;
  * query synchronises on them and returns the summed elapsed time and the launch count. */

@@ -1,0 +1,491 @@
+// Device-resident biconjugate / symmetric Lanczos on several chains, one hipGraph per step.
+//
+// The recurrence is the reference's run_FT (Modules/DynamicalLanczos.py:5843-6372, without
+// re-orthogonalisation): with p_norm_k = s_norm_k / c_(k-1),
+//   a_k   = p_norm_k <p_k, L q_k>
+//   r_k   = L q_k - a_k q_k - c_(k-1) q_(k-1)
+//   s_k   = L^T p_k - a_k p_k - b_(k-1) (p_norm_(k-1) / p_norm_k) p_(k-1)
+//   b_k   = |r_k|,  s_norm_(k+1) = |s_k| p_norm_k,  c_k = <s_k/|s_k|, r_k/b_k> s_norm_(k+1)
+//   q_(k+1) = r_k / b_k,  p_(k+1) = s_k / |s_k|
+// (Wigner symmetric form: p = q, L^T p = L q, dot products weighted by the mask that counts
+// off-diagonal pair entries twice, Py:5790-5839.) All scalars stay in FP64 device memory;
+// each step appends a_k, b_k, c_k, s_norm_(k+1) to device arrays at a device step counter,
+// so one captured graph serves every step. The host truncates at the first |b| or |c|
+// below 1e-12 (the reference's stopping rule, Py:6144-6153).
+#include <string>
+#include <vector>
+
+#include "tdscha_internal.h"
+#include "../../include/tdscha_hip.h"
+
+namespace tdscha {
+namespace {
+
+constexpr int kDotBlocks = 256;  // fixed partial count: reductions have a fixed order
+
+// state per chain: [0] p_norm, [1] old p_norm, [2] c_(k-1), [3] b_(k-1), [4] a_k
+enum { ST_PN = 0, ST_OPN, ST_CL, ST_BL, ST_A, ST_N };
+
+struct ChainVecs {
+  double *q, *p, *qp, *pp;   // current and previous vectors
+  double *Lq, *pL;           // L q and L^T p (then overwritten by r and s)
+};
+
+__device__ inline double block_sum(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+  __syncthreads();
+  return t;  // valid in thread 0
+}
+
+// partial[c][blk] = sum over the block's strided range of x . y (* mask)
+__global__ __launch_bounds__(256) void k_dot_partial(const double *const *x, const double *const *y,
+                                                     const double *__restrict__ mask, long n,
+                                                     double *partial) {
+  __shared__ double sh[4];
+  const int c = blockIdx.y;
+  const double *xc = x[c], *yc = y[c];
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    acc = fma(xc[i], mask ? yc[i] * mask[i] : yc[i], acc);
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[c * kDotBlocks + blockIdx.x] = acc;
+}
+
+__device__ inline double sum_partials(const double *part, double *sh) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kDotBlocks; i += blockDim.x) v += part[i];
+  return block_sum(v, sh);
+}
+
+// a_k = p_norm <p, Lq>; record it.
+__global__ __launch_bounds__(256) void k_finish_a(const double *partial, double *state,
+                                                  double *a_all, const int *step, int max_steps) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double d = sum_partials(partial + c * kDotBlocks, sh);
+  if (threadIdx.x == 0) {
+    double *st = state + c * ST_N;
+    const double a = d * st[ST_PN];
+    st[ST_A] = a;
+    const int k = *step;
+    if (k < max_steps) a_all[c * max_steps + k] = a;
+  }
+}
+
+// r = Lq - a q - c_last qp ; s = pL - a p - b_last (opn / pn) pp  (in place), and the
+// partial sums of r.r, s.s, s.r
+__global__ __launch_bounds__(256) void k_residuals(const ChainVecs *vecs, const double *state,
+                                                   const double *__restrict__ mask, long n,
+                                                   int symmetric, double *partial) {
+  __shared__ double sh[4];
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  const double *st = state + c * ST_N;
+  const double a = st[ST_A], cl = st[ST_CL], bl = st[ST_BL];
+  const double ratio = st[ST_PN] != 0.0 ? st[ST_OPN] / st[ST_PN] : 0.0;
+  double rr = 0.0, ss = 0.0, sr = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double r = v.Lq[i] - a * v.q[i] - cl * v.qp[i];
+    const double s = symmetric ? r : v.pL[i] - a * v.p[i] - bl * ratio * v.pp[i];
+    v.Lq[i] = r;
+    if (!symmetric) v.pL[i] = s;
+    const double m = mask ? mask[i] : 1.0;
+    rr = fma(r, r * m, rr);
+    ss = fma(s, s * m, ss);
+    sr = fma(s, r * m, sr);
+  }
+  rr = block_sum(rr, sh);
+  ss = block_sum(ss, sh);
+  sr = block_sum(sr, sh);
+  if (threadIdx.x == 0) {
+    partial[(c * 3 + 0) * kDotBlocks + blockIdx.x] = rr;
+    partial[(c * 3 + 1) * kDotBlocks + blockIdx.x] = ss;
+    partial[(c * 3 + 2) * kDotBlocks + blockIdx.x] = sr;
+  }
+}
+
+// b, c, s_norm; record them and advance the per-chain state. scal[c] = (1/b, 1/|s|).
+__global__ __launch_bounds__(256) void k_finish_bc(const double *partial, double *state,
+                                                   double *b_all, double *c_all, double *s_all,
+                                                   double *scal, int *step, int n_chains,
+                                                   int max_steps) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double rr = sum_partials(partial + (c * 3 + 0) * kDotBlocks, sh);
+  const double ss = sum_partials(partial + (c * 3 + 1) * kDotBlocks, sh);
+  const double sr = sum_partials(partial + (c * 3 + 2) * kDotBlocks, sh);
+  if (threadIdx.x == 0) {
+    double *st = state + c * ST_N;
+    const double pn = st[ST_PN];
+    const double s_raw = sqrt(ss);
+    const double s_norm = s_raw * pn;
+    const double b = sqrt(rr);
+    const double cc = (sr / (s_raw * b)) * s_norm;
+    const int k = *step;
+    if (k < max_steps) {
+      b_all[c * max_steps + k] = b;
+      c_all[c * max_steps + k] = cc;
+      s_all[c * max_steps + k] = s_norm;
+    }
+    st[ST_OPN] = pn;
+    st[ST_PN] = s_norm / cc;
+    st[ST_CL] = cc;
+    st[ST_BL] = b;
+    scal[2 * c] = 1.0 / b;
+    scal[2 * c + 1] = 1.0 / s_raw;
+  }
+}
+
+__global__ void k_advance_step(int *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// qp <- q, q <- r / b, pp <- p, p <- s / |s|; optionally store the new pair in the basis
+__global__ __launch_bounds__(256) void k_shift(const ChainVecs *vecs, const double *scal, long n,
+                                               int symmetric, double *basisQ, double *basisP,
+                                               const int *step, int max_steps) {
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  const double ib = scal[2 * c], is = scal[2 * c + 1];
+  const int k = *step;  // already advanced: index of the new vector
+  double *bq = (basisQ && k <= max_steps) ? basisQ + ((size_t)c * (max_steps + 1) + k) * n : nullptr;
+  double *bp = (basisP && k <= max_steps) ? basisP + ((size_t)c * (max_steps + 1) + k) * n : nullptr;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double nq = v.Lq[i] * ib;
+    const double np = symmetric ? nq : v.pL[i] * is;
+    v.qp[i] = v.q[i];
+    v.q[i] = nq;
+    if (!symmetric) {
+      v.pp[i] = v.p[i];
+      v.p[i] = np;
+    }
+    if (bq) bq[i] = nq;
+    if (bp) bp[i] = np;
+  }
+}
+
+// q = p = psi / sqrt(psi . psi (* mask)), previous vectors zero, state reset
+__global__ __launch_bounds__(256) void k_start(const ChainVecs *vecs, const double *const *psi,
+                                               const double *partial, long n, int symmetric,
+                                               double *state, double *basisQ, double *basisP,
+                                               int max_steps) {
+  __shared__ double sh[4];
+  __shared__ double inv;
+  const int c = blockIdx.y;
+  const double d = sum_partials(partial + c * kDotBlocks, sh);
+  if (threadIdx.x == 0) inv = 1.0 / sqrt(d);
+  __syncthreads();
+  const ChainVecs v = vecs[c];
+  double *bq = basisQ ? basisQ + (size_t)c * (max_steps + 1) * n : nullptr;
+  double *bp = basisP ? basisP + (size_t)c * (max_steps + 1) * n : nullptr;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double x = psi[c][i] * inv;
+    v.q[i] = x;
+    v.qp[i] = 0.0;
+    if (!symmetric) {
+      v.p[i] = x;
+      v.pp[i] = 0.0;
+    }
+    if (bq) bq[i] = x;
+    if (bp) bp[i] = x;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double *st = state + c * ST_N;
+    st[ST_PN] = 1.0;   // s_norm_0 / c_(-1) = 1
+    st[ST_OPN] = 0.0;
+    st[ST_CL] = 0.0;
+    st[ST_BL] = 0.0;
+    st[ST_A] = 0.0;
+  }
+}
+
+__global__ void k_zero_int(int *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+}  // namespace
+}  // namespace tdscha
+
+using namespace tdscha;
+
+struct tdscha_lanczos {
+  tdscha_ensemble *ens = nullptr;
+  int prec = 0, nc = 0, symmetric = 0, max_steps = 0, keep_basis = 0, device = 0;
+  long n_psi = 0;
+  std::vector<int> flags;
+  std::vector<double> shifts;
+  std::vector<void *> allocs;
+  double *vec = nullptr;         // 6 vectors per chain
+  ChainVecs *dvecs = nullptr;
+  const double **dpsi_in = nullptr, **dpsi_p = nullptr;  // pointer tables for the dots
+  double *mask = nullptr;
+  double *partial = nullptr, *state = nullptr, *scal = nullptr;
+  double *a_all = nullptr, *b_all = nullptr, *c_all = nullptr, *s_all = nullptr;
+  double *basisQ = nullptr, *basisP = nullptr;
+  int *step = nullptr;
+  std::vector<const double *> apply_in;
+  std::vector<double *> apply_out;
+  std::vector<int> apply_flags;
+  std::vector<double> apply_shifts;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+int lz_alloc(tdscha_lanczos *z, void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return 11;
+  }
+  z->allocs.push_back(*p);
+  return 0;
+}
+
+// One Lanczos step enqueued on st (the body that is captured in the graph).
+int enqueue_step(tdscha_lanczos *z, hipStream_t st) {
+  const long n = z->n_psi;
+  const int C = z->nc;
+  const int napply = z->symmetric ? C : 2 * C;
+  int rc = tdscha_apply_full_L_batched(z->ens, z->prec, napply, z->apply_flags.data(),
+                                       z->apply_shifts.data(), z->apply_in.data(),
+                                       z->apply_out.data(), st);
+  if (rc) return rc;
+  // a = p_norm <p, L q>  (symmetric: <q, L q>_mask)
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, C), dim3(256), 0, st,
+                     (const double *const *)z->dpsi_p, (const double *const *)(z->dpsi_in + C),
+                     z->symmetric ? z->mask : (const double *)nullptr, n, z->partial);
+  hipLaunchKernelGGL(k_finish_a, dim3(C), dim3(256), 0, st, z->partial, z->state, z->a_all, z->step,
+                     z->max_steps);
+  hipLaunchKernelGGL(k_residuals, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, z->state,
+                     z->symmetric ? z->mask : (const double *)nullptr, n, z->symmetric, z->partial);
+  hipLaunchKernelGGL(k_finish_bc, dim3(C), dim3(256), 0, st, z->partial, z->state, z->b_all,
+                     z->c_all, z->s_all, z->scal, z->step, C, z->max_steps);
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, st, z->step);
+  hipLaunchKernelGGL(k_shift, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, z->scal, n,
+                     z->symmetric, z->basisQ, z->basisP, z->step, z->max_steps);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("Lanczos step launch failed: ") + hipGetErrorString(e));
+    return 12;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precision, int n_chains,
+                          const int *flags, const double *shifts, int symmetric, int max_steps,
+                          int keep_basis, const double *mask) {
+  if (!out || !ens || !flags || !shifts || n_chains < 1 || max_steps < 1) {
+    set_error("bad argument to tdscha_lanczos_create");
+    return 1;
+  }
+  if (2 * n_chains > TDSCHA_MAX_CHAINS) {
+    set_error("too many chains");
+    return 1;
+  }
+  if (symmetric && !mask) {
+    set_error("the symmetric (Wigner) Lanczos needs the dot-product mask");
+    return 1;
+  }
+  int dims[6];
+  tdscha_ensemble_dims(ens, dims);
+  // every buffer of the batched application exists before a step is captured
+  int rc0 = tdscha_ensemble_reserve(ens, symmetric ? n_chains : 2 * n_chains);
+  if (rc0) return rc0;
+  const long n = dims[0];
+  tdscha_lanczos *z = new tdscha_lanczos();
+  z->ens = ens;
+  z->prec = precision;
+  z->nc = n_chains;
+  z->symmetric = symmetric;
+  z->max_steps = max_steps;
+  z->keep_basis = keep_basis;
+  z->n_psi = n + n * (n + 1);
+  z->flags.assign(flags, flags + n_chains);
+  z->shifts.assign(shifts, shifts + n_chains);
+  (void)hipGetDevice(&z->device);
+  const long N = z->n_psi;
+  const int C = n_chains;
+  int rc = 0;
+  rc |= lz_alloc(z, (void **)&z->vec, sizeof(double) * 6 * C * N);
+  rc |= lz_alloc(z, (void **)&z->dvecs, sizeof(ChainVecs) * C);
+  rc |= lz_alloc(z, (void **)&z->dpsi_in, sizeof(double *) * 2 * C);
+  rc |= lz_alloc(z, (void **)&z->dpsi_p, sizeof(double *) * C);
+  rc |= lz_alloc(z, (void **)&z->partial, sizeof(double) * 3 * C * kDotBlocks);
+  rc |= lz_alloc(z, (void **)&z->state, sizeof(double) * ST_N * C);
+  rc |= lz_alloc(z, (void **)&z->scal, sizeof(double) * 2 * C);
+  rc |= lz_alloc(z, (void **)&z->a_all, sizeof(double) * C * max_steps);
+  rc |= lz_alloc(z, (void **)&z->b_all, sizeof(double) * C * max_steps);
+  rc |= lz_alloc(z, (void **)&z->c_all, sizeof(double) * C * max_steps);
+  rc |= lz_alloc(z, (void **)&z->s_all, sizeof(double) * C * max_steps);
+  rc |= lz_alloc(z, (void **)&z->step, sizeof(int));
+  if (symmetric) rc |= lz_alloc(z, (void **)&z->mask, sizeof(double) * N);
+  if (keep_basis) {
+    rc |= lz_alloc(z, (void **)&z->basisQ, sizeof(double) * C * (max_steps + 1) * N);
+    if (!symmetric) rc |= lz_alloc(z, (void **)&z->basisP, sizeof(double) * C * (max_steps + 1) * N);
+  }
+  if (rc) { tdscha_lanczos_destroy(z); return rc; }
+  std::vector<ChainVecs> hv(C);
+  std::vector<const double *> hin(2 * C), hp(C);
+  z->apply_in.resize(symmetric ? C : 2 * C);
+  z->apply_out.resize(symmetric ? C : 2 * C);
+  z->apply_flags.resize(symmetric ? C : 2 * C);
+  z->apply_shifts.resize(symmetric ? C : 2 * C);
+  for (int c = 0; c < C; ++c) {
+    double *base = z->vec + (size_t)6 * c * N;
+    hv[c] = ChainVecs{base, symmetric ? base : base + N, base + 2 * N, symmetric ? base + 2 * N : base + 3 * N,
+                      base + 4 * N, base + 5 * N};
+    z->apply_in[c] = hv[c].q;
+    z->apply_out[c] = hv[c].Lq;
+    z->apply_flags[c] = flags[c] & ~TDSCHA_OP_TRANSPOSE;
+    z->apply_shifts[c] = shifts[c];
+    if (!symmetric) {
+      z->apply_in[C + c] = hv[c].p;
+      z->apply_out[C + c] = hv[c].pL;
+      z->apply_flags[C + c] = flags[c] | TDSCHA_OP_TRANSPOSE;
+      z->apply_shifts[C + c] = shifts[c];
+    }
+    hp[c] = hv[c].p;
+    hin[c] = hv[c].q;
+    hin[C + c] = hv[c].Lq;
+  }
+  hipError_t e = hipMemcpy(z->dvecs, hv.data(), sizeof(ChainVecs) * C, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(z->dpsi_in, hin.data(), sizeof(double *) * 2 * C, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(z->dpsi_p, hp.data(), sizeof(double *) * C, hipMemcpyHostToDevice);
+  if (e == hipSuccess && symmetric) e = hipMemcpy(z->mask, mask, sizeof(double) * N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error(std::string("Lanczos set-up copy failed: ") + hipGetErrorString(e));
+    tdscha_lanczos_destroy(z);
+    return 10;
+  }
+  *out = z;
+  return 0;
+}
+
+int tdscha_lanczos_destroy(tdscha_lanczos *z) {
+  if (!z) return 0;
+  if (z->exec) (void)hipGraphExecDestroy(z->exec);
+  if (z->graph) (void)hipGraphDestroy(z->graph);
+  if (z->graph_stream) (void)hipStreamDestroy(z->graph_stream);
+  for (void *p : z->allocs) (void)hipFree(p);
+  delete z;
+  return 0;
+}
+
+int tdscha_lanczos_start(tdscha_lanczos *z, const double *const *psi0, void *stream) {
+  if (!z || !psi0) { set_error("NULL argument"); return 1; }
+  hipStream_t st = (hipStream_t)stream;
+  const int C = z->nc;
+  const double **dpsi = nullptr;
+  // the starting vectors are read through a device pointer table
+  std::vector<const double *> hp(psi0, psi0 + C);
+  const double **tab = nullptr;
+  if (hipMalloc(&tab, sizeof(double *) * C) != hipSuccess) { set_error("hipMalloc failed"); return 11; }
+  z->allocs.push_back(tab);
+  dpsi = tab;
+  if (hipMemcpy(dpsi, hp.data(), sizeof(double *) * C, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("pointer table copy failed");
+    return 10;
+  }
+  hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, C), dim3(256), 0, st,
+                     (const double *const *)dpsi, (const double *const *)dpsi,
+                     z->symmetric ? z->mask : (const double *)nullptr, z->n_psi, z->partial);
+  hipLaunchKernelGGL(k_start, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs,
+                     (const double *const *)dpsi, z->partial, z->n_psi, z->symmetric, z->state,
+                     z->basisQ, z->basisP, z->max_steps);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, z->step);
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_error(std::string("Lanczos start failed: ") + hipGetErrorString(e));
+    return 10;
+  }
+  return 0;
+}
+
+int tdscha_lanczos_run(tdscha_lanczos *z, int n_steps, int use_graph, void *stream) {
+  if (!z || n_steps < 0) { set_error("bad argument"); return 1; }
+  hipStream_t st = (hipStream_t)stream;
+  if (!use_graph) {
+    for (int i = 0; i < n_steps; ++i) {
+      int rc = enqueue_step(z, st);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  if (!z->exec) {
+    // capture one step on a private stream (default priority), instantiate once
+    if (hipStreamCreateWithFlags(&z->graph_stream, hipStreamNonBlocking) != hipSuccess) {
+      set_error("stream creation failed");
+      return 10;
+    }
+    int prof = tdscha_profile_enable(0);  // no event records inside a captured graph
+    (void)prof;
+    hipError_t e = hipStreamBeginCapture(z->graph_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      set_error(std::string("stream capture failed: ") + hipGetErrorString(e));
+      return 10;
+    }
+    int rc = enqueue_step(z, z->graph_stream);
+    e = hipStreamEndCapture(z->graph_stream, &z->graph);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      set_error(std::string("end capture failed: ") + hipGetErrorString(e));
+      return 10;
+    }
+    e = hipGraphInstantiate(&z->exec, z->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      set_error(std::string("graph instantiation failed: ") + hipGetErrorString(e));
+      return 10;
+    }
+  }
+  for (int i = 0; i < n_steps; ++i) {
+    hipError_t e = hipGraphLaunch(z->exec, st);
+    if (e != hipSuccess) {
+      set_error(std::string("graph launch failed: ") + hipGetErrorString(e));
+      return 10;
+    }
+  }
+  return 0;
+}
+
+int tdscha_lanczos_read(tdscha_lanczos *z, int chain, double *a, double *b, double *c,
+                        double *s_norm, int *steps_done) {
+  if (!z || chain < 0 || chain >= z->nc || !steps_done) { set_error("bad argument"); return 1; }
+  int k = 0;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(&k, z->step, sizeof(int), hipMemcpyDeviceToHost);
+  if (k > z->max_steps) k = z->max_steps;
+  const size_t off = (size_t)chain * z->max_steps, bytes = sizeof(double) * k;
+  if (e == hipSuccess && a) e = hipMemcpy(a, z->a_all + off, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && b) e = hipMemcpy(b, z->b_all + off, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && c) e = hipMemcpy(c, z->c_all + off, bytes, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && s_norm) e = hipMemcpy(s_norm, z->s_all + off, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_error(std::string("reading the coefficients failed: ") + hipGetErrorString(e));
+    return 10;
+  }
+  *steps_done = k;
+  return 0;
+}
+
+int tdscha_lanczos_basis(tdscha_lanczos *z, int chain, const double **Q, const double **P) {
+  if (!z || chain < 0 || chain >= z->nc) { set_error("bad argument"); return 1; }
+  const size_t off = (size_t)chain * (z->max_steps + 1) * z->n_psi;
+  if (Q) *Q = z->basisQ ? z->basisQ + off : nullptr;
+  if (P) *P = z->basisP ? z->basisP + off : (z->symmetric && z->basisQ ? z->basisQ + off : nullptr);
+  return 0;
+}
+
+}  // extern "C"

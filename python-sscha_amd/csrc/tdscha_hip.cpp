@@ -487,6 +487,12 @@ int tdscha_ensemble_destroy(tdscha_ensemble *h) {
   return 0;
 }
 
+int tdscha_ensemble_reserve(tdscha_ensemble *h, int n_chains) {
+  if (!h || n_chains < 1 || n_chains > TDSCHA_MAX_CHAINS) { set_error("bad argument"); return 1; }
+  TD_CHECK(hipSetDevice(h->e.device));
+  return ensure_chains(h, n_chains);
+}
+
 int tdscha_ensemble_dims(const tdscha_ensemble *h, int *d) {
   if (!h || !d) { set_error("NULL argument"); return 1; }
   d[0] = h->e.n; d[1] = h->e.N; d[2] = h->e.N_sym; d[3] = h->e.s_batch;

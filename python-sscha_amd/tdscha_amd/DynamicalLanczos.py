@@ -287,11 +287,19 @@ class Lanczos(object):
     # ------------------------------------------------------------------ Lanczos
     def run_FT(self, n_iter, save_dir=None, save_each=5, verbose=True, n_rep_orth=0, n_ortho=10,
                flush_output=True, debug=False, prefix="LANCZOS", run_simm=False, optimized=False,
-               full_gs=False):
-        """Biconjugate (or Wigner symmetric) Lanczos, Py:5843-6372, Krylov vectors on device."""
+               full_gs=False, device_loop=False, graph=True):
+        """Biconjugate (or Wigner symmetric) Lanczos, Py:5843-6372, Krylov vectors on device.
+
+        device_loop=True runs the whole recurrence on the GPU (tdscha_lanczos_*, one hipGraph
+        per step when graph=True); it needs a fresh start, n_rep_orth = 0 and no full_gs.
+        """
         import torch
         self.verbose = verbose
         ens = self.device_ensemble
+        if device_loop:
+            if len(self.a_coeffs) or n_rep_orth or full_gs:
+                raise ValueError("device_loop needs a fresh start, n_rep_orth=0, full_gs=False")
+            return self._run_device_loop(n_iter, run_simm, graph, save_dir, prefix)
         dev = ens.device
         psi_host = np.asarray(self.psi, dtype=TYPE_DP)
         psi_norm = np.sum(psi_host ** 2)
@@ -437,6 +445,33 @@ class Lanczos(object):
         self.s_norm = s_norm_l
         self.basis_Q = [q.cpu().numpy() for q in Q]
         self.basis_P = [p.cpu().numpy() for p in P]
+        return self
+
+    def _run_device_loop(self, n_iter, run_simm, graph, save_dir, prefix):
+        from .lanczos_device import DeviceLanczos
+        ens = self.device_ensemble
+        if run_simm and not self.use_wigner:
+            raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
+        psi = np.asarray(self.psi, dtype=TYPE_DP)
+        if np.isnan(np.sum(psi ** 2)) or np.sum(psi ** 2) == 0:
+            raise ValueError("Error, you must initialize a perturbation to start the Lanczos.")
+        fl = self._flags(False) & ~L.OP_TRANSPOSE
+        dl = DeviceLanczos(ens, 1, self.precision, [fl], [float(self.shift_value)],
+                           symmetric=run_simm, max_steps=n_iter, keep_basis=True,
+                           mask=self.mask_dot_wigner() if run_simm else None)
+        dl.start([psi])
+        dl.run(n_iter, graph=graph)
+        a, b, c, s = dl.coefficients(0)
+        Q, P = dl.basis(0, len(b))
+        self.a_coeffs, self.b_coeffs, self.c_coeffs = list(a), list(b), list(c)
+        self.s_norm = [1.0] + list(s)
+        self.basis_Q = [q for q in Q.cpu().numpy()]
+        self.basis_P = [p for p in P.cpu().numpy()]
+        dl.close()
+        if save_dir is not None:
+            if not os.path.exists(save_dir):
+                os.makedirs(save_dir)
+            self.save_status("%s/%s_STEP%d" % (save_dir, prefix, len(a)))
         return self
 
     # ------------------------------------------------------------------ Green function

@@ -47,6 +47,16 @@ SIGNATURES = {
         _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
     "tdscha_apply_full_L_batched": (ctypes.c_int, [
         _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+    "tdscha_ensemble_reserve": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tdscha_lanczos_create": (ctypes.c_int, [
+        ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, _dp]),
+    "tdscha_lanczos_destroy": (ctypes.c_int, [_vp]),
+    "tdscha_lanczos_start": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
+    "tdscha_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "tdscha_lanczos_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "tdscha_lanczos_basis": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp),
+                                            ctypes.POINTER(_vp)]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

@@ -1,0 +1,130 @@
+"""Device-resident Lanczos over several chains (tdscha_lanczos_* in include/tdscha_hip.h).
+
+Each step is one batched L application plus the FP64 recurrence of run_FT
+(Modules/DynamicalLanczos.py:5843-6372) on the GPU, captured once as a hipGraph and
+replayed; the host reads the coefficients when the run is over.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .ensemble import _stream_handle
+
+EPSILON = 1e-12   # the reference's stopping threshold (Py:121, Py:6144-6153)
+
+
+def truncate(a, b, c, s_norm=None):
+    """Cut the coefficients where the reference's run_FT stops: a_k is kept, b_k / c_k of the
+    first step with |b_k| or |c_k| below 1e-12 are not (Py:6143-6153)."""
+    a, b, c = (np.asarray(x, dtype=np.float64) for x in (a, b, c))
+    bad = np.nonzero((np.abs(b) < EPSILON) | (np.abs(c) < EPSILON) | ~np.isfinite(b) |
+                     ~np.isfinite(c))[0]
+    if len(bad):
+        i = bad[0]
+        return a[:i + 1], b[:i], c[:i], (None if s_norm is None else np.asarray(s_norm)[:i])
+    return a, b, c, (None if s_norm is None else np.asarray(s_norm))
+
+
+class DeviceLanczos:
+    """n_chains Lanczos chains on one DeviceEnsemble.
+
+    flags[c], shifts[c]: TDSCHA_OP_* bits (WIGNER, REVERSE, IGNORE_*) and shift of chain c.
+    symmetric: the Wigner symmetric form (run_simm), needs mask (mask_dot_wigner).
+    """
+
+    def __init__(self, dev_ens, n_chains=1, precision="fp64", flags=None, shifts=None,
+                 symmetric=False, max_steps=100, keep_basis=False, mask=None):
+        self.ens = dev_ens
+        self.torch = dev_ens.torch
+        self.nc = int(n_chains)
+        self.max_steps = int(max_steps)
+        self.symmetric = bool(symmetric)
+        self.keep_basis = bool(keep_basis)
+        n = dev_ens.n
+        self.n_psi = n + n * (n + 1)
+        flags = [0] * self.nc if flags is None else list(flags)
+        shifts = [0.0] * self.nc if shifts is None else list(shifts)
+        fl = (ctypes.c_int * self.nc)(*[int(f) for f in flags])
+        sh = (ctypes.c_double * self.nc)(*[float(s) for s in shifts])
+        m = None
+        if symmetric:
+            if mask is None:
+                raise ValueError("the symmetric Lanczos needs the Wigner mask")
+            self._mask = np.ascontiguousarray(mask, dtype=np.float64)
+            m = self._mask.ctypes.data_as(L._dp)
+        h = ctypes.c_void_p()
+        rc = L.lib().tdscha_lanczos_create(ctypes.byref(h), dev_ens._h, L.PRECISIONS[precision],
+                                           self.nc, fl, sh, int(symmetric), self.max_steps,
+                                           int(keep_basis), m)
+        L.check(rc, "tdscha_lanczos_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().tdscha_lanczos_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def start(self, psis):
+        t = self.torch
+        if len(psis) != self.nc:
+            raise ValueError("expected %d starting vectors" % self.nc)
+        self._psi0 = [self.ens._dev(p, (self.n_psi,)) for p in psis]
+        pp = (ctypes.c_void_p * self.nc)(*[p.data_ptr() for p in self._psi0])
+        rc = L.lib().tdscha_lanczos_start(self._h, pp, _stream_handle(t, self.ens.device))
+        L.check(rc, "tdscha_lanczos_start")
+
+    def run(self, n_steps, graph=True):
+        rc = L.lib().tdscha_lanczos_run(self._h, int(n_steps), int(bool(graph)),
+                                        _stream_handle(self.torch, self.ens.device))
+        L.check(rc, "tdscha_lanczos_run")
+
+    def coefficients(self, chain=0, truncated=True):
+        k = ctypes.c_int()
+        a = np.zeros(self.max_steps)
+        b = np.zeros(self.max_steps)
+        c = np.zeros(self.max_steps)
+        s = np.zeros(self.max_steps)
+        rc = L.lib().tdscha_lanczos_read(self._h, int(chain), a.ctypes.data_as(L._dp),
+                                         b.ctypes.data_as(L._dp), c.ctypes.data_as(L._dp),
+                                         s.ctypes.data_as(L._dp), ctypes.byref(k))
+        L.check(rc, "tdscha_lanczos_read")
+        a, b, c, s = a[:k.value], b[:k.value], c[:k.value], s[:k.value]
+        if truncated:
+            return truncate(a, b, c, s)
+        return a, b, c, s
+
+    def basis(self, chain, steps):
+        """(Q, P) device views of the first steps + 1 stored vectors of a chain."""
+        if not self.keep_basis:
+            raise ValueError("created without keep_basis")
+        q = ctypes.c_void_p()
+        p = ctypes.c_void_p()
+        L.check(L.lib().tdscha_lanczos_basis(self._h, int(chain), ctypes.byref(q), ctypes.byref(p)),
+                "tdscha_lanczos_basis")
+        count = (steps + 1) * self.n_psi
+        shape = (steps + 1, self.n_psi)
+        # views of library-owned device memory, valid while this object lives
+        Q = _DeviceView(q.value, count, self.ens.device, self.torch).tensor().reshape(shape)
+        P = _DeviceView(p.value, count, self.ens.device, self.torch).tensor().reshape(shape)
+        return Q, P
+
+
+class _DeviceView:
+    """A torch tensor over library-owned device memory (via __cuda_array_interface__)."""
+
+    def __init__(self, ptr, count, device, torch):
+        self.__cuda_array_interface__ = {
+            "shape": (count,), "typestr": "<f8", "data": (ptr, False), "version": 2,
+            "strides": None}
+        self._device = device
+        self._torch = torch
+
+    def tensor(self):
+        return self._torch.as_tensor(self, device=self._device)

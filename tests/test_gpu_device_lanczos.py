@@ -1,0 +1,110 @@
+"""The device-resident Lanczos (tdscha_lanczos_*): recurrence on the GPU, one hipGraph per
+step, several chains per pass over the symmetries."""
+import numpy as np
+import pytest
+
+import fixtures
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_ens(ens, precisions=("fp64",)):
+    from tdscha_amd.ensemble import DeviceEnsemble, SymmetryBlocks
+    s = ens.sym
+    return DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T,
+                          SymmetryBlocks(s.symmetries, s.N_degeneracy, s.degenerate_space,
+                                         s.block_id), precisions=precisions)
+
+
+def test_device_loop_matches_reference_run(mode10):
+    """Same coefficients as the reference's 10-step run (first 8 steps, see the host test)
+    and the KAT c_4, with the graph replayed once per step."""
+    from tdscha_amd.lanczos_device import DeviceLanczos
+    st = fixtures.load_npz("lanczos_std.npz")
+    dev = _dev_ens(mode10)
+    dl = DeviceLanczos(dev, 1, "fp64", max_steps=10, keep_basis=True)
+    dl.start([st["Q"][0]])
+    dl.run(10, graph=True)
+    a, b, c, s = dl.coefficients(0)
+    assert len(a) == 10
+    assert abs(c[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
+    assert np.max(np.abs(a - st["a"]) / np.abs(st["a"])) < 1e-8
+    assert np.max(np.abs(c[:8] - st["c"][:8]) / np.abs(st["c"][:8])) < 1e-6
+    assert np.max(np.abs(s[:8] - st["s_norm"][1:9]) / np.abs(st["s_norm"][1:9])) < 1e-6
+    Q, P = dl.basis(0, 8)
+    assert np.max(np.abs(Q.cpu().numpy() - st["Q"][:9])) < 1e-6 * np.abs(st["Q"]).max()
+
+
+def test_graph_replays_equal_eager_steps(mode10):
+    """Graph replays give bit-identical coefficients to the same steps launched eagerly
+    (run several times, with host work and synchronisation between the replays)."""
+    import torch
+    from tdscha_amd.lanczos_device import DeviceLanczos
+    st = fixtures.load_npz("lanczos_std.npz")
+    dev = _dev_ens(mode10)
+    res = []
+    for graph in (False, True, True):
+        dl = DeviceLanczos(dev, 1, "fp64", max_steps=12)
+        dl.start([st["Q"][3]])
+        for _ in range(3):
+            dl.run(4, graph=graph)
+            torch.cuda.synchronize()
+            _ = np.linalg.norm(np.ones(10))
+        res.append(dl.coefficients(0, truncated=False))
+        dl.close()
+    for r in res[1:]:
+        for x, y in zip(r, res[0]):
+            assert np.array_equal(x, y)
+
+
+def test_device_loop_wigner_symmetric(mode10):
+    from tdscha_amd import _lib as L
+    from tdscha_amd.lanczos_device import DeviceLanczos
+    st = fixtures.load_npz("lanczos_wigner.npz")
+    dev = _dev_ens(mode10)
+    dl = DeviceLanczos(dev, 1, "fp64", flags=[L.OP_WIGNER], symmetric=True, max_steps=10,
+                       mask=orc.mask_dot_wigner(45))
+    dl.start([st["Q"][0]])
+    dl.run(10)
+    a, b, c, _ = dl.coefficients(0)
+    assert np.max(np.abs(a - st["a"]) / np.abs(st["a"])) < 1e-8
+    assert np.max(np.abs(b[:8] - st["b"][:8]) / np.abs(st["b"][:8])) < 1e-6
+    assert np.allclose(b, c, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_chains_advance_independently(precision):
+    """Three chains in one run give the coefficients of three single-chain runs."""
+    from tdscha_amd.lanczos_device import DeviceLanczos
+    ens = fixtures.random_ensemble(51, 40, 80, 6)
+    dev = _dev_ens(ens, precisions=("fp64", precision))
+    rng = np.random.default_rng(52)
+    psis = [rng.standard_normal(orc.psi_len(40)) for _ in range(3)]
+    shifts = [0.0, 1e-7, 0.0]
+    many = DeviceLanczos(dev, 3, precision, shifts=shifts, max_steps=8)
+    many.start(psis)
+    many.run(8)
+    for ch in range(3):
+        one = DeviceLanczos(dev, 1, precision, shifts=[shifts[ch]], max_steps=8)
+        one.start([psis[ch]])
+        one.run(8)
+        for x, y in zip(many.coefficients(ch), one.coefficients(0)):
+            if x is None:
+                continue
+            assert np.allclose(x, y, rtol=1e-9, atol=0), ch
+        one.close()
+
+
+def test_lanczos_class_device_loop(mode10):
+    from tdscha_amd import DynamicalLanczos as DL
+    st = fixtures.load_npz("lanczos_std.npz")
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64)
+    lz.set_ensemble(mode10.X, mode10.Y, mode10.w, mode10.rho, mode10.T)
+    s = mode10.sym
+    lz.set_symmetries(s.symmetries, s.N_degeneracy, s.degenerate_space, s.block_id)
+    lz.init()
+    lz.prepare_mode(10)
+    lz.run_FT(10, verbose=False, device_loop=True)
+    assert abs(lz.c_coeffs[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
+    assert len(lz.basis_Q) == len(lz.b_coeffs) + 1
