@@ -3,9 +3,11 @@
 
 Workload (default): synthetic rock-salt 6x6x6 supercell, 432 atoms, 1293 modes, 20000
 configurations, the full Fm-3m x translations image set (10368 symmetries), T = 300 K.
-A step is one full L application (tdscha_apply_full_L: harmonic part, pre-transform,
-the ensemble average over all 20000 x 10368 images, post-transform) on a seeded random
-unit psi, with every input already resident in HBM.
+A step is one batched L application (tdscha_apply_full_L_batched) to the pair a Lanczos
+step needs, L q and L^T p, on seeded random unit vectors: for each chain the harmonic part,
+the pre-transform, the ensemble average over all 20000 x 10368 images and the
+post-transform, with both chains sharing every pass over the symmetries. Every input is
+resident in HBM; value counts L applications (2 per step) per second.
 
 Prints one JSON line. `roofline` is measured live with HIP events around every launch of
 each kernel group over the timed steps; `cpu_baseline` times the reference's own C trio
@@ -76,7 +78,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rocksalt6", choices=sorted(WORKLOADS))
     ap.add_argument("--configs", type=int, default=0, help="override the number of configs")
-    ap.add_argument("--precision", default="fp64", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--precision", default="bf16x3", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--chains", type=int, default=2,
+                    help="vectors per batched application (alternating L and L^T)")
     ap.add_argument("--s-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-sample-syms", type=int, default=-1,
@@ -109,12 +113,19 @@ def main():
                           s_batch=args.s_batch)
     log("device ensemble: %.2f GB, s_batch=%d" % (dens.nbytes / 1e9, dens.s_batch))
     g = torch.Generator().manual_seed(3)
-    psi = torch.randn(n + n * (n + 1), generator=g, dtype=torch.float64)
-    psi = (psi / psi.norm()).to(dev)
-    out = torch.empty_like(psi)
+    psis = []
+    for _ in range(args.chains):
+        v = torch.randn(n + n * (n + 1), generator=g, dtype=torch.float64)
+        psis.append((v / v.norm()).to(dev))
+    outs = [torch.empty_like(p) for p in psis]
+    flags = [L.OP_TRANSPOSE * (c % 2) for c in range(args.chains)]
+    shifts = [0.0] * args.chains
+
+    def step():
+        dens.apply_full_L_batched(psis, flags, shifts, precision=args.precision, outs=outs)
 
     for k in range(args.warmup):
-        dens.apply_full_L(psi, precision=args.precision, out=out)
+        step()
         torch.cuda.synchronize()
         log("warmup %d done" % k)
     L.lib().tdscha_profile_reset()
@@ -122,13 +133,13 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        dens.apply_full_L(psi, precision=args.precision, out=out)
+        step()
         torch.cuda.synchronize()
         log("step %d done (%.1f s)" % (k, time.perf_counter() - t0))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     L.lib().tdscha_profile_enable(0)
-    if not torch.isfinite(out).all():
+    if not all(bool(torch.isfinite(o).all()) for o in outs):
         raise RuntimeError("non-finite output")
 
     groups = {name: L.profile_query(gid) for gid, name in L.KGROUP_NAMES.items()}
@@ -140,13 +151,13 @@ def main():
     share = {"gemm1": 2.0 / 6.0, "gemm2": 4.0 / 6.0}
     dom = max(("gemm1", "gemm2"), key=lambda k: groups[k][0])
     ms, cnt = groups[dom]
-    per_launch_flop = flop_l * share[dom] / (cnt / args.steps)
+    per_launch_flop = flop_l * args.chains * share[dom] / (cnt / args.steps)
     achieved = per_launch_flop / (ms / cnt * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.precision]
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                 "kernel": "k_%s_%s" % (dom, "f64" if args.precision == "fp64" else args.precision)}
-    value = args.steps / elapsed
+    value = args.chains * args.steps / elapsed
     fp64_ref = None
     if args.precision != "fp64" and args.fp64_sample_syms != 0:
         # the plain FP64 path on the first k symmetries: its cost is linear in the number of
@@ -158,10 +169,10 @@ def main():
         del dens
         torch.cuda.empty_cache()
         d64 = DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sub, s_batch=args.s_batch)
-        d64.apply_full_L(psi, precision="fp64", out=out)
+        d64.apply_full_L(psis[0], precision="fp64", out=outs[0])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        d64.apply_full_L(psi, precision="fp64", out=out)
+        d64.apply_full_L(psis[0], precision="fp64", out=outs[0])
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
         v64 = 1.0 / (t64 * ens.n_syms / k)
@@ -190,8 +201,10 @@ def main():
         "dtype": {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "rock-salt %dx%dx%d supercell, %d atoms, %d modes, %d configs, "
-                               "%d symmetry images per config, full L application (%s)" % (
-                                   Lc, Lc, Lc, 2 * Lc ** 3, n, N, ens.n_syms, args.precision),
+                               "%d symmetry images per config, %d-chain batched L application "
+                               "(%s)" % (Lc, Lc, Lc, 2 * Lc ** 3, n, N, ens.n_syms, args.chains,
+                                         args.precision),
+                   "chains": args.chains,
                    "n_modes": n, "n_configs": N, "n_syms": ens.n_syms,
                    "flop_per_L_application": flop_l,
                    "achieved_TFLOPs_whole_L": round(flop_l * value / 1e12, 3)},

@@ -535,6 +535,103 @@ __global__ __launch_bounds__(256) void k_gemm2_bf16x3(Gemm2BfArgs g) {
       }
 }
 
+// ---------------------------------------------------------------- f_av back-projection
+// V[s][j] = sum_k YT[j][k] e1_s[k] + XtT[j][k] e2_s[k] in FP64 MFMA for every precision:
+// an NT product (rows j x slots s, k contiguous in both operands), 64 rows x 32 slots per
+// workgroup (waves 2 x 2, each 32 rows x 16 slots), split over k in chunks of kFavChunk
+// configurations; the partial sums Vp[chunk][s][j] are added in a fixed order afterwards.
+constexpr int FV_S = 32;
+constexpr int FV_LDB = FV_S + 16;
+
+__global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
+                                                  const double *__restrict__ XtT,
+                                                  const double *__restrict__ W,
+                                                  double *__restrict__ Vp, int Np, int npd,
+                                                  int sb) {
+  __shared__ double As[2][BK * LDS_LD];
+  __shared__ double Bs[2][BK * FV_LDB];
+  const int j0 = blockIdx.x * BM, kc = blockIdx.y, q0 = blockIdx.z * FV_S;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1;
+  const int lr = l & 15, lg = l >> 4;
+  const int k_begin = kc * kFavChunk;
+  const int k_end = min(Np, k_begin + kFavChunk);
+  const int KT = (k_end - k_begin) / BK;
+  // B slice: 32 slots x 16 k, two consecutive k per thread
+  const int bq = threadIdx.x >> 3, bk = (threadIdx.x & 7) * 2;
+  const bool bq_ok = q0 + bq < sb;
+
+  v4d acc[2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[ti][r] = 0.0;
+
+  for (int part = 0; part < 2; ++part) {
+    const double *M = part ? XtT : YT;
+    const double *E = W + (size_t)(part ? 3 : 2) * Np;   // e2 or e1 of slot 0
+    auto load_b = [&](int k0, double (&rb)[2]) {
+      if (bq_ok) {
+        const double *p = E + (size_t)(q0 + bq) * 4 * Np + k0 + bk;
+        rb[0] = p[0];
+        rb[1] = p[1];
+      } else {
+        rb[0] = rb[1] = 0.0;
+      }
+    };
+    auto store_b = [&](double *lds, const double (&rb)[2]) {
+      lds[bk * FV_LDB + bq] = rb[0];
+      lds[(bk + 1) * FV_LDB + bq] = rb[1];
+    };
+    double ra[4], rb[2];
+    load_slice(M, Np, j0, k_begin, ra);
+    load_b(k_begin, rb);
+    __syncthreads();   // the previous part's last reads of As / Bs are done
+    store_slice(As[0], ra);
+    store_b(Bs[0], rb);
+    __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < KT) {
+        load_slice(M, Np, j0, k_begin + (kt + 1) * BK, ra);
+        load_b(k_begin + (kt + 1) * BK, rb);
+      }
+#pragma unroll
+      for (int kq = 0; kq < BK / 4; ++kq) {
+        const int k = kq * 4 + lg;
+        const double a0 = As[cur][k * LDS_LD + wr * 32 + lr];
+        const double a1 = As[cur][k * LDS_LD + wr * 32 + 16 + lr];
+        const double b = Bs[cur][k * FV_LDB + wc * 16 + lr];
+        acc[0] = mfma(a0, b, acc[0]);
+        acc[1] = mfma(a1, b, acc[1]);
+      }
+      if (kt + 1 < KT) {
+        store_slice(As[cur ^ 1], ra);
+        store_b(Bs[cur ^ 1], rb);
+      }
+      __syncthreads();
+    }
+  }
+  const int q = q0 + wc * 16 + lr;
+  if (q < sb) {
+    double *out = Vp + ((size_t)kc * sb + q) * npd + j0;
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[wr * 32 + ti * 16 + out_row<double>(lg, r)] = acc[ti][r];
+  }
+}
+
+// V[s][j] = sum over chunks of Vp[chunk][s][j], chunks in increasing order.
+__global__ void k_fav_reduce(const double *__restrict__ Vp, double *__restrict__ V, int n_chunks,
+                             size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  double acc = 0.0;
+  for (int c = 0; c < n_chunks; ++c) acc += Vp[(size_t)c * count + i];
+  V[i] = acc;
+}
+
 }  // namespace
 
 void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st) {
@@ -546,13 +643,13 @@ void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st)
     Gemm1BfArgs g;
     const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
     g.Xh = (const uint16_t *)e.X_hi; g.Xl = (const uint16_t *)e.X_lo;
-    g.Ysh = (const uint16_t *)e.Ys_lp; g.Ysl = (const uint16_t *)e.Ys_lp + plane;
+    g.Ysh = e.Ys_bf; g.Ysl = e.Ys_bf + plane;
     g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
     g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1_bf16x3, dim3(e.Np / LB, sb), dim3(256), 0, st, g);
   } else if (prec == kFP32) {
     Gemm1Args<float> g;
-    g.A = e.X_f; g.Ys = (const float *)e.Ys_lp;
+    g.A = e.X_f; g.Ys = e.Ys_f32;
     g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
     g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
@@ -581,6 +678,15 @@ void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st) {
     g.XtT = e.XtT; g.YT = e.YT; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
     hipLaunchKernelGGL(k_gemm2<double>, dim3(e.npd / BM, e.npd / BN, sb), dim3(256), 0, st, g);
   }
+}
+
+void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
+  const int n_chunks = (e.Np + kFavChunk - 1) / kFavChunk;
+  dim3 grid(e.npd / BM, n_chunks, (sb + FV_S - 1) / FV_S);
+  hipLaunchKernelGGL(k_fav_mfma, grid, dim3(256), 0, st, e.YT, e.XtT, e.W, e.Vp, e.Np, e.npd, sb);
+  const size_t count = (size_t)sb * e.npd;
+  hipLaunchKernelGGL(k_fav_reduce, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, e.Vp,
+                     e.V, n_chunks, count);
 }
 
 }  // namespace tdscha

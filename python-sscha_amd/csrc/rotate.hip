@@ -172,9 +172,18 @@ __global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B
   d2v += (size_t)ch * npd * npd;
   const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
   const int cnt = dA * W;
+  // the images are added one at a time onto the running sum, so the result does not depend
+  // on how the symmetries are split into passes
   double acc[kPer];
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) acc[j] = 0.0;
+  for (int j = 0; j < kPer; ++j) {
+    const int idx = threadIdx.x + 256 * j;
+    acc[j] = 0.0;
+    if (idx < cnt) {
+      const int r = idx / W, c = idx - r * W;
+      acc[j] = d2v[(size_t)(offA + r) * npd + g0 + c];
+    }
+  }
   for (int t = 0; t < ps.ns; ++t) {
     const int sl = ch * ps.ns + t;
     const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
@@ -222,7 +231,7 @@ __global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B
     const int idx = threadIdx.x + 256 * j;
     if (idx < cnt) {
       const int r = idx / W, c = idx - r * W;
-      d2v[(size_t)(offA + r) * npd + g0 + c] += acc[j];
+      d2v[(size_t)(offA + r) * npd + g0 + c] = acc[j];
     }
   }
 }
@@ -252,46 +261,6 @@ __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G
 
 // V[s][j] = sum_i YT[j][i] e1[s][i] + XtT[j][i] e2[s][i]
 // Block: 4 waves x 4 rows each = 16 rows j; lanes stride over configs i.
-constexpr int kFavRowsPerWave = 4;
-constexpr int kFavMaxS = 16;
-__global__ __launch_bounds__(256) void k_fav(const double *__restrict__ YT,
-                                             const double *__restrict__ XtT,
-                                             const double *__restrict__ W, double *__restrict__ V,
-                                             int Np, int npd, int s_first, int s_count) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j0 = (blockIdx.x * 4 + wave) * kFavRowsPerWave;
-  double acc[kFavRowsPerWave][kFavMaxS];
-#pragma unroll
-  for (int r = 0; r < kFavRowsPerWave; ++r)
-#pragma unroll
-    for (int q = 0; q < kFavMaxS; ++q) acc[r][q] = 0.0;
-  for (int i = lane; i < Np; i += 64) {
-    double yv[kFavRowsPerWave], xv[kFavRowsPerWave];
-#pragma unroll
-    for (int r = 0; r < kFavRowsPerWave; ++r) {
-      yv[r] = YT[(size_t)(j0 + r) * Np + i];
-      xv[r] = XtT[(size_t)(j0 + r) * Np + i];
-    }
-#pragma unroll
-    for (int q = 0; q < kFavMaxS; ++q) {
-      if (q < s_count) {
-        const double *Ws = W + (size_t)(s_first + q) * 4 * Np;
-        double e1 = Ws[2 * (size_t)Np + i], e2 = Ws[3 * (size_t)Np + i];
-#pragma unroll
-        for (int r = 0; r < kFavRowsPerWave; ++r) acc[r][q] = fma(yv[r], e1, fma(xv[r], e2, acc[r][q]));
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < kFavRowsPerWave; ++r)
-#pragma unroll
-    for (int q = 0; q < kFavMaxS; ++q) {
-      double v = acc[r][q];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-      if (lane == 0 && q < s_count) V[(size_t)(s_first + q) * npd + j0 + r] = v;
-    }
-}
-
 // f_acc[chain][a] += sum over the chain's slots of sum_k S_s[a'][k] V[slot][off_A + k]
 __global__ void k_fav_rot_acc(const double *__restrict__ V, double *__restrict__ f_acc,
                               const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
@@ -383,7 +352,7 @@ void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t 
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
   dim3 grid(e.n_blocks, e.n_groups, ps.slots());
   hipLaunchKernelGGL(k_rot1_fused, grid, dim3(256), 0, st, e.Ysym,
-                     prec == kFP64 ? e.Ys : (double *)nullptr, e.Ys_lp, prec, plane, e.S,
+                     prec == kFP64 ? e.Ys : (double *)nullptr, e.ys_lp(prec), prec, plane, e.S,
                      e.dblocks, e.dgroups, e.meta, e.npd, e.sym_stride, ps);
 }
 
@@ -397,14 +366,6 @@ void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {
   dim3 grid((e.npd + 255) / 256, ps.slots());
   hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, e.R1p, e.G, e.S, e.meta, e.fups, e.fpsi,
                      e.n, e.npd, e.sym_stride, ps);
-}
-
-void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
-  dim3 grid(e.npd / (4 * kFavRowsPerWave));
-  for (int q0 = 0; q0 < sb; q0 += kFavMaxS) {
-    int cnt = sb - q0 < kFavMaxS ? sb - q0 : kFavMaxS;
-    hipLaunchKernelGGL(k_fav, grid, dim3(256), 0, st, e.YT, e.XtT, e.W, e.V, e.Np, e.npd, q0, cnt);
-  }
 }
 
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st) {
@@ -432,9 +393,11 @@ void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st) {
   const size_t count = (size_t)sb * e.npd * e.npd;
   if (prec == kBF16x3) {
     // hi plane [0, plane), lo plane [plane, 2 plane): convert hi and lo of the first `count`
-    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, plane, prec);
+    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.ys_lp(prec), plane,
+                       prec);
   } else {
-    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.Ys_lp, count, prec);
+    hipLaunchKernelGGL(k_convert, grid1(count, 256), dim3(256), 0, st, e.Ys, e.ys_lp(prec), count,
+                       prec);
   }
 }
 

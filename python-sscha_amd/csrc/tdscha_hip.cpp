@@ -225,14 +225,15 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   rc |= dev_alloc(h, &e.W, sb * 4 * e.Np);
   rc |= dev_alloc(h, &e.B, sb * nn);
   rc |= dev_alloc(h, &e.V, sb * e.npd);
-  if (e.prec_mask & ((1 << kFP32) | (1 << kBF16x3))) {
-    char *lp = nullptr;
-    rc |= dev_alloc(h, &lp, sb * nn * 4);  // fp32 plane or bf16 hi + lo planes
-    e.Ys_lp = lp;
-  }
+  rc |= dev_alloc(h, &e.Vp, (size_t)((e.Np + kFavChunk - 1) / kFavChunk) * sb * e.npd);
+  e.Ys_f32 = nullptr;
+  e.Ys_bf = nullptr;
+  if (e.prec_mask & (1 << kFP32)) rc |= dev_alloc(h, &e.Ys_f32, sb * nn);
+  if (e.prec_mask & (1 << kBF16x3)) rc |= dev_alloc(h, &e.Ys_bf, 2 * sb * nn);
   if (rc) return rc;
   if (hipMemset(e.Ys, 0, sb * nn * 8) != hipSuccess ||
-      (e.Ys_lp && hipMemset(e.Ys_lp, 0, sb * nn * 4) != hipSuccess) ||
+      (e.Ys_f32 && hipMemset(e.Ys_f32, 0, sb * nn * 4) != hipSuccess) ||
+      (e.Ys_bf && hipMemset(e.Ys_bf, 0, sb * nn * 4) != hipSuccess) ||
       hipMemset(e.B, 0, sb * nn * 8) != hipSuccess) {
     set_error("workspace initialisation failed");
     return 10;

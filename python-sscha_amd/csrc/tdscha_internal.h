@@ -27,6 +27,7 @@ constexpr int kPadCols = 64;   // mode padding (GEMM tiles along modes)
 constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 
 constexpr int kRotTile = 64;   // widest block / column group of the fused rotations
+constexpr int kFavChunk = 1024; // configurations per split-k chunk of the f_av product
 
 enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
 
@@ -92,11 +93,18 @@ struct Ensemble {
   // workspace (per pass of s_batch slots)
   double *Tw = nullptr;       // [s_batch x npd x npd]  rotation intermediate
   double *Ys = nullptr;       // [s_batch x npd x npd]  S_s^T Ysym S_s
-  void *Ys_lp = nullptr;      // reduced-precision Ys (bf16 hi+lo or fp32)
+  // reduced-precision copies of Ys, one buffer per precision (their layouts differ, and the
+  // padding each GEMM reads as part of its k range must stay zero in its own layout)
+  float *Ys_f32 = nullptr;    // [s_batch x npd x npd]
+  uint16_t *Ys_bf = nullptr;  // hi plane [s_batch x npd x npd], then the lo plane
+  void *ys_lp(int prec) const {
+    return prec == kBF16x3 ? (void *)Ys_bf : prec == kFP32 ? (void *)Ys_f32 : nullptr;
+  }
   double *G = nullptr;        // [s_batch x 2 x npd]  g1 = f_ups*S^T R1, g2 = S^T R1 / f_psi
   double *W = nullptr;        // [s_batch x 4 x Np]  c1, c2, e1, e2 per image
   double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
+  double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
 };
 
 void set_error(const std::string &msg);
@@ -112,7 +120,6 @@ void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st);
 void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st);
 // d2v_acc[chain] += sum over the chain's slots of S (B_s + B_s^T) S^T, fused
 void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st);
-void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st);
 void launch_zero(double *p, size_t count, hipStream_t st);
 void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
@@ -127,6 +134,8 @@ void launch_post(const Ensemble &e, int chain, const double *psi, double *out, i
                  double shift, hipStream_t st);
 
 // ---- GEMMs (gemm.hip) ----
+// V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
+void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 // GEMM1: per image (config i, symmetry s) the quadratic forms
 //   alpha = x_i^T Ys_s x_i, beta = ft_i^T Ys_s x_i, gamma = x_i . g1_s, delta = ft_i . g2_s
 // and the per-image weights (c1, c2, e1, e2) written to W.
