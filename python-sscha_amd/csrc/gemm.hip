@@ -15,10 +15,7 @@
 //
 // FP64 / FP32: v_mfma_{f64,f32}_16x16x4, 64 x 64 workgroup tile, each wave a 32 x 32
 // sub-tile (2 x 2 MFMA blocks), BK = 16 slices double-buffered through LDS as [k][m].
-// BF16x3: a*b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi with v_mfma_f32_16x16x32_bf16 and FP32
-// accumulation, 128 x 128 workgroup tile, each wave a 64 x 64 sub-tile (4 x 4 MFMA blocks),
-// BK = 32 slices staged as [m][k] rows (row stride 40 bf16: conflict-free 16-byte reads).
-// Every epilogue (row dots, weights) is FP64.
+// The BF16x3 forms are in gemm_bf16.hip. Every epilogue (row dots, weights) is FP64.
 #include "tdscha_internal.h"
 
 namespace tdscha {
@@ -288,253 +285,6 @@ __global__ __launch_bounds__(256) void k_gemm2(Gemm2Args<T> g) {
       }
 }
 
-// ---------------------------------------------------------------- BF16 x3 (16x16x32)
-constexpr int LB = 128;      // workgroup tile (rows and columns)
-constexpr int LK = 32;       // k slice
-constexpr int LLD = LK + 8;  // LDS row stride in bf16 (80 bytes)
-
-
-struct Gemm1BfArgs {
-  const uint16_t *Xh, *Xl;    // [Np x npd]
-  const uint16_t *Ysh, *Ysl;  // [sb x npd x npd]
-  const double *X, *Ft, *rho, *G;
-  double *W;
-  int Np, npd;
-  int do_q;
-  double kf, kr, ky;
-};
-
-struct Gemm2BfArgs {
-  const uint16_t *XtTh, *XtTl;  // [npd x Np]
-  const float *YTf, *XtTf;      // [npd x Np]
-  const double *W;
-  double *B;
-  int Np, npd;
-};
-
-// Two 16-byte chunks per thread cover a 128 x 32 bf16 slice: rows t/4 and t/4 + 64.
-__device__ inline void load_bf_slice(const uint16_t *__restrict__ M, int ld, int rows, int r0,
-                                     int k0, uint4 (&reg)[2]) {
-  const int t = threadIdx.x;
-  const int row = t >> 2, ch = (t & 3) * 8;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int r = r0 + row + 64 * h;
-    reg[h] = (r < rows) ? *reinterpret_cast<const uint4 *>(M + (size_t)r * ld + k0 + ch)
-                        : make_uint4(0, 0, 0, 0);
-  }
-}
-
-__device__ inline void store_bf_slice(uint16_t *lds, const uint4 (&reg)[2]) {
-  const int t = threadIdx.x;
-  const int row = t >> 2, ch = (t & 3) * 8;
-#pragma unroll
-  for (int h = 0; h < 2; ++h) *reinterpret_cast<uint4 *>(lds + (row + 64 * h) * LLD + ch) = reg[h];
-}
-
-__device__ inline v8bf lds_frag(const uint16_t *lds, int row, int kg) {
-  uint4 u = *reinterpret_cast<const uint4 *>(lds + row * LLD + kg * 8);
-  return __builtin_bit_cast(v8bf, u);
-}
-
-// 48 MFMAs of one 32-deep slice for this wave's 4 x 4 block of 16 x 16 tiles.
-__device__ inline void mma_bf16x3(const uint16_t *Ah, const uint16_t *Al, const uint16_t *Bh,
-                                  const uint16_t *Bl, int wr, int wc, v4f (&acc)[4][4]) {
-  const int l = threadIdx.x & 63;
-  const int lr = l & 15, lg = l >> 4;
-  v8bf bh[4], bl[4];
-#pragma unroll
-  for (int tj = 0; tj < 4; ++tj) {
-    bh[tj] = lds_frag(Bh, wc * 64 + tj * 16 + lr, lg);
-    bl[tj] = lds_frag(Bl, wc * 64 + tj * 16 + lr, lg);
-  }
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti) {
-    const v8bf ah = lds_frag(Ah, wr * 64 + ti * 16 + lr, lg);
-    const v8bf al = lds_frag(Al, wr * 64 + ti * 16 + lr, lg);
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) {
-      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
-      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
-      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_gemm1_bf16x3(Gemm1BfArgs g) {
-  __shared__ uint16_t Ah[LB * LLD], Al[LB * LLD], Bh[LB * LLD], Bl[LB * LLD];
-  __shared__ double red[2][LB][4];
-  const int i0 = blockIdx.x * LB;
-  const int sl = blockIdx.y;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1;
-  const int lr = l & 15, lg = l >> 4;
-  const size_t ys_off = (size_t)sl * g.npd * g.npd;
-  const double *g1 = g.G + (size_t)sl * 2 * g.npd;
-  const double *g2 = g1 + g.npd;
-
-  for (int t = threadIdx.x; t < 2 * LB * 4; t += 256) (&red[0][0][0])[t] = 0.0;
-  __syncthreads();
-
-  const int KT = g.npd / LK;
-  for (int c0 = 0; c0 < g.npd; c0 += LB) {
-    v4f acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
-    if (g.do_q) {
-      uint4 rah[2], ral[2], rbh[2], rbl[2];
-      load_bf_slice(g.Xh, g.npd, g.Np, i0, 0, rah);
-      load_bf_slice(g.Xl, g.npd, g.Np, i0, 0, ral);
-      load_bf_slice(g.Ysh + ys_off, g.npd, g.npd, c0, 0, rbh);
-      load_bf_slice(g.Ysl + ys_off, g.npd, g.npd, c0, 0, rbl);
-      for (int kt = 0; kt < KT; ++kt) {
-        __syncthreads();
-        store_bf_slice(Ah, rah);
-        store_bf_slice(Al, ral);
-        store_bf_slice(Bh, rbh);
-        store_bf_slice(Bl, rbl);
-        __syncthreads();
-        if (kt + 1 < KT) {
-          const int k0 = (kt + 1) * LK;
-          load_bf_slice(g.Xh, g.npd, g.Np, i0, k0, rah);
-          load_bf_slice(g.Xl, g.npd, g.Np, i0, k0, ral);
-          load_bf_slice(g.Ysh + ys_off, g.npd, g.npd, c0, k0, rbh);
-          load_bf_slice(g.Ysl + ys_off, g.npd, g.npd, c0, k0, rbl);
-        }
-        mma_bf16x3(Ah, Al, Bh, Bl, wr, wc, acc);
-      }
-    }
-    // epilogue: FP64 row dots of this column tile, reduced over the 16 column lanes
-#pragma unroll
-    for (int ti = 0; ti < 4; ++ti) {
-      double p[4][4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj) {
-        const int c = c0 + wc * 64 + tj * 16 + lr;
-        if (c < g.npd) {
-          const double gg1 = g1[c], gg2 = g2[c];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int m = wr * 64 + ti * 16 + 4 * lg + r;
-            const size_t o = (size_t)(i0 + m) * g.npd + c;
-            const double x = g.X[o], ft = g.Ft[o];
-            const double q = (double)acc[ti][tj][r];
-            p[r][0] = fma(x, q, p[r][0]);
-            p[r][1] = fma(ft, q, p[r][1]);
-            p[r][2] = fma(x, gg1, p[r][2]);
-            p[r][3] = fma(ft, gg2, p[r][3]);
-          }
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          double v = p[r][q];
-#pragma unroll
-          for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-          if (lr == 0) red[wc][wr * 64 + ti * 16 + 4 * lg + r][q] += v;
-        }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < LB) {
-    const int m = threadIdx.x, i = i0 + m;
-    write_weights(g.W + (size_t)sl * 4 * g.Np, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
-                  red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
-                  red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
-  }
-}
-
-// G^T slice (rows c0.., k0..k0+31) formed in FP32 and split into bf16 hi / lo.
-__device__ inline void load_g_bf_slice(const Gemm2BfArgs &g, const double *c1, const double *c2,
-                                       int r0, int k0, uint4 (&rh)[2], uint4 (&rl)[2]) {
-  const int t = threadIdx.x;
-  const int row = t >> 2, ch = (t & 3) * 8;
-  float a[8], b[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    a[j] = (float)c1[k0 + ch + j];
-    b[j] = (float)(0.5 * c2[k0 + ch + j]);
-  }
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int r = r0 + row + 64 * h;
-    uint16_t hi[8], lo[8];
-    if (r < g.npd) {
-      const float4 *py = reinterpret_cast<const float4 *>(g.YTf + (size_t)r * g.Np + k0 + ch);
-      const float4 *px = reinterpret_cast<const float4 *>(g.XtTf + (size_t)r * g.Np + k0 + ch);
-      float4 y0 = py[0], y1 = py[1], x0 = px[0], x1 = px[1];
-      float yv[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
-      float xv[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const float v = fmaf(a[j], yv[j], b[j] * xv[j]);
-        hi[j] = bf16_rne(v);
-        lo[j] = bf16_rne(v - bf16_to_f(hi[j]));
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) hi[j] = lo[j] = 0;
-    }
-    rh[h] = make_uint4(hi[0] | ((uint32_t)hi[1] << 16), hi[2] | ((uint32_t)hi[3] << 16),
-                       hi[4] | ((uint32_t)hi[5] << 16), hi[6] | ((uint32_t)hi[7] << 16));
-    rl[h] = make_uint4(lo[0] | ((uint32_t)lo[1] << 16), lo[2] | ((uint32_t)lo[3] << 16),
-                       lo[4] | ((uint32_t)lo[5] << 16), lo[6] | ((uint32_t)lo[7] << 16));
-  }
-}
-
-__global__ __launch_bounds__(256) void k_gemm2_bf16x3(Gemm2BfArgs g) {
-  __shared__ uint16_t Ah[LB * LLD], Al[LB * LLD], Bh[LB * LLD], Bl[LB * LLD];
-  const int m0 = blockIdx.x * LB, c0 = blockIdx.y * LB, sl = blockIdx.z;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1;
-  const int lr = l & 15, lg = l >> 4;
-  const double *c1 = g.W + (size_t)sl * 4 * g.Np;
-  const double *c2 = c1 + g.Np;
-  v4f acc[4][4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
-
-  const int KT = g.Np / LK;
-  uint4 rah[2], ral[2], rbh[2], rbl[2];
-  load_bf_slice(g.XtTh, g.Np, g.npd, m0, 0, rah);
-  load_bf_slice(g.XtTl, g.Np, g.npd, m0, 0, ral);
-  load_g_bf_slice(g, c1, c2, c0, 0, rbh, rbl);
-  for (int kt = 0; kt < KT; ++kt) {
-    __syncthreads();
-    store_bf_slice(Ah, rah);
-    store_bf_slice(Al, ral);
-    store_bf_slice(Bh, rbh);
-    store_bf_slice(Bl, rbl);
-    __syncthreads();
-    if (kt + 1 < KT) {
-      const int k0 = (kt + 1) * LK;
-      load_bf_slice(g.XtTh, g.Np, g.npd, m0, k0, rah);
-      load_bf_slice(g.XtTl, g.Np, g.npd, m0, k0, ral);
-      load_g_bf_slice(g, c1, c2, c0, k0, rbh, rbl);
-    }
-    mma_bf16x3(Ah, Al, Bh, Bl, wr, wc, acc);
-  }
-  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
-#pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wr * 64 + ti * 16 + 4 * lg + r;
-        const int c = c0 + wc * 64 + tj * 16 + lr;
-        if (m < g.npd && c < g.npd) Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
-      }
-}
-
 // ---------------------------------------------------------------- f_av back-projection
 // V[s][j] = sum_k YT[j][k] e1_s[k] + XtT[j][k] e2_s[k] in FP64 MFMA for every precision:
 // an NT product (rows j x slots s, k contiguous in both operands), 64 rows x 32 slots per
@@ -640,13 +390,7 @@ void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st)
   const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
   const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
   if (prec == kBF16x3) {
-    Gemm1BfArgs g;
-    const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
-    g.Xh = (const uint16_t *)e.X_hi; g.Xl = (const uint16_t *)e.X_lo;
-    g.Ysh = e.Ys_bf; g.Ysl = e.Ys_bf + plane;
-    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
-    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
-    hipLaunchKernelGGL(k_gemm1_bf16x3, dim3(e.Np / LB, sb), dim3(256), 0, st, g);
+    launch_gemm1_bf16x3(e, sb, what, st);
   } else if (prec == kFP32) {
     Gemm1Args<float> g;
     g.A = e.X_f; g.Ys = e.Ys_f32;
@@ -664,11 +408,7 @@ void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st)
 
 void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st) {
   if (prec == kBF16x3) {
-    Gemm2BfArgs g;
-    g.XtTh = (const uint16_t *)e.XtT_hi; g.XtTl = (const uint16_t *)e.XtT_lo;
-    g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
-    const int t = (e.npd + LB - 1) / LB;
-    hipLaunchKernelGGL(k_gemm2_bf16x3, dim3(t, t, sb), dim3(256), 0, st, g);
+    launch_gemm2_bf16x3(e, sb, st);
   } else if (prec == kFP32) {
     Gemm2Args<float> g;
     g.XtT = e.XtT_f; g.YT = e.YT_f; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;

@@ -1,0 +1,395 @@
+// BF16x3 contractions of the ensemble L-application (gfx950).
+//
+// a*b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi with hi = RNE_bf16(v), lo = RNE_bf16(v - hi):
+// three v_mfma_f32_16x16x32_bf16 per operand pair, FP32 accumulation, FP64 epilogues.
+//
+// Both contractions are "NT" products C[m][c] = sum_k A[m][k] B[c][k] with k-contiguous
+// operand rows, tiled R x 128 per workgroup (4 waves as 2 x 2, each R/2 x 64 = R/32 x 4
+// MFMA blocks; R = 256 for GEMM1, 192 for GEMM2, whose operand formation needs the
+// registers), k in slices of 32 staged through LDS as [row][k] with an 80-byte row stride
+// (conflict-free 16-byte fragment reads). The next slice is fetched into registers while
+// the current one is multiplied. Sub-blocks past the valid rows / columns of a ragged tile
+// issue no MFMAs.
+//
+// Workgroups are numbered so that each XCD (they are dealt out round-robin by workgroup id)
+// runs a contiguous range of tiles, slot-major: the concurrent workgroups of an XCD work on
+// the same symmetry image and share its operand slices in that XCD's L2.
+//
+// GEMM1 (per image s, configs i):  Q = X Ys_s, with the row sums of the tile
+//     alpha_i = sum_c X_ic Q_ic, beta_i = sum_c Ft_ic Q_ic, gamma_i = sum_c X_ic g1_c,
+//     delta_i = sum_c Ft_ic g2_c
+//   written per column tile to P[s][ct][4][Np]; k_weights_bf16 adds the column tiles in
+//   order and forms c1, c2, e1, e2 (LanczosFunctions.c:2195-2231, 2374-2407, 2687-2725).
+// GEMM2 (per image s):  B_s = X~^T G_s with G_s^T[c][k] = c1_k Y^T[c][k] + c2_k/2 X~^T[c][k]
+//   formed in FP32 from the FP32 operand copies and split while it is staged.
+#include "tdscha_internal.h"
+
+namespace tdscha {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+
+constexpr int TM1 = 256;         // GEMM1 workgroup tile rows (configurations)
+constexpr int TM2 = 192;         // GEMM2 workgroup tile rows (modes)
+constexpr int TN = kBfTileN;     // workgroup tile columns (128)
+constexpr int TK = 32;           // k slice
+constexpr int LDK = TK + 8;      // LDS row stride in bf16
+constexpr int WN = 4;            // 16 x 16 column blocks per wave (64 columns)
+
+template <int R> struct Stage {
+  uint16_t ah[R * LDK], al[R * LDK], bh[TN * LDK], bl[TN * LDK];
+};
+
+// Workgroup id -> logical tile id, so that XCD x runs logical ids [x * per, (x + 1) * per).
+// The grid has 8 * per workgroups; ids past `total` return.
+__device__ inline int xcd_tile(int total) {
+  const int per = (total + 7) / 8;
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+// Pairs of FP32 values split into packed bf16 hi / lo (v_cvt_pk_bf16_f32, nearest even).
+__device__ inline void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+  const v2f f = {a, b};
+  const v2bf h = __builtin_convertvector(f, v2bf);
+  const v2f hf = __builtin_convertvector(h, v2f);
+  const v2bf l = __builtin_convertvector(f - hf, v2bf);
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// A slice of R rows x 32 k of one bf16 plane: R / 64 16-byte chunks per thread, rows
+// t/4 + 64 h (zero past `rows`).
+template <int R>
+__device__ inline void load_rows(const uint16_t *__restrict__ P, size_t ld, int rows, int r0,
+                                 int k0, uint4 (&reg)[R / 64]) {
+  const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < R / 64; ++h) {
+    const int r = r0 + row + 64 * h;
+    reg[h] = r < rows ? *reinterpret_cast<const uint4 *>(P + (size_t)r * ld + k0 + kc)
+                      : make_uint4(0, 0, 0, 0);
+  }
+}
+
+template <int R>
+__device__ inline void store_rows(uint16_t *lds, const uint4 (&reg)[R / 64]) {
+  const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < R / 64; ++h)
+    *reinterpret_cast<uint4 *>(lds + (row + 64 * h) * LDK + kc) = reg[h];
+}
+
+__device__ inline v8bf frag(const uint16_t *lds, int row, int kg) {
+  return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(lds + row * LDK + kg * 8));
+}
+
+// 3 x (mi x nj) MFMAs of one slice for this wave; mi / nj = blocks with valid rows / columns.
+template <int R, int WM = R / 32>
+__device__ inline void mma_slice(const Stage<R> &s, int wr, int wc, int mi, int nj,
+                                 v4f (&acc)[WM][WN]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  v8bf bh[WN], bl[WN];
+#pragma unroll
+  for (int tj = 0; tj < WN; ++tj) {
+    bh[tj] = frag(s.bh, wc * 64 + tj * 16 + lr, lg);
+    bl[tj] = frag(s.bl, wc * 64 + tj * 16 + lr, lg);
+  }
+#pragma unroll
+  for (int ti = 0; ti < WM; ++ti) {
+    if (ti < mi) {
+      const v8bf ah = frag(s.ah, wr * (R / 2) + ti * 16 + lr, lg);
+      const v8bf al = frag(s.al, wr * (R / 2) + ti * 16 + lr, lg);
+#pragma unroll
+      for (int tj = 0; tj < WN; ++tj) {
+        if (tj < nj) {
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+__device__ inline int clamp_blocks(int valid, int start, int cap) {
+  int b = (valid - start + 15) / 16;
+  return b < 0 ? 0 : (b > cap ? cap : b);
+}
+
+// ------------------------------------------------------------------------------ GEMM1
+struct Gemm1Args {
+  const uint16_t *Xh, *Xl;    // [Np x npd]
+  const uint16_t *Yh, *Yl;    // [s_batch x npd x npd] planes of the rotated Y1
+  const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
+  const double *G;            // [s_batch x 2 x npd]
+  double *P;                  // [sb x n_ct x 4 x Np] row sums per column tile
+  int Np, npd, sb, n_it, n_ct, do_q;
+};
+
+__global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
+  constexpr int TM = TM1, WM = TM1 / 32;
+  __shared__ Stage<TM1> sm;
+  __shared__ double red[2][TM][4];
+  const int total = g.sb * g.n_it * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  // slot-major, then row tile, column tile fastest
+  const int sl = id / (g.n_it * g.n_ct);
+  const int rest = id - sl * g.n_it * g.n_ct;
+  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  const int i0 = it * TM, c0 = ct * TN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1, lr = l & 15, lg = l >> 4;
+  const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
+  const int mi = WM;   // Np is a multiple of the row tile
+  const size_t plane_off = (size_t)sl * g.npd * g.npd;
+
+  v4f acc[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  if (g.do_q) {
+    const int KT = g.npd / TK;
+    uint4 rah[TM / 64], ral[TM / 64], rbh[TN / 64], rbl[TN / 64];
+    load_rows<TM>(g.Xh, g.npd, g.Np, i0, 0, rah);
+    load_rows<TM>(g.Xl, g.npd, g.Np, i0, 0, ral);
+    load_rows<TN>(g.Yh + plane_off, g.npd, g.npd, c0, 0, rbh);
+    load_rows<TN>(g.Yl + plane_off, g.npd, g.npd, c0, 0, rbl);
+    for (int kt = 0; kt < KT; ++kt) {
+      __syncthreads();
+      store_rows<TM>(sm.ah, rah);
+      store_rows<TM>(sm.al, ral);
+      store_rows<TN>(sm.bh, rbh);
+      store_rows<TN>(sm.bl, rbl);
+      __syncthreads();
+      if (kt + 1 < KT) {
+        const int k0 = (kt + 1) * TK;
+        load_rows<TM>(g.Xh, g.npd, g.Np, i0, k0, rah);
+        load_rows<TM>(g.Xl, g.npd, g.Np, i0, k0, ral);
+        load_rows<TN>(g.Yh + plane_off, g.npd, g.npd, c0, k0, rbh);
+        load_rows<TN>(g.Yl + plane_off, g.npd, g.npd, c0, k0, rbl);
+      }
+      mma_slice<TM>(sm, wr, wc, mi, nj, acc);
+    }
+  }
+
+  // epilogue: FP64 row sums over this tile's columns, one 16-row block at a time
+  const double *g1 = g.G + (size_t)sl * 2 * g.npd, *g2 = g1 + g.npd;
+#pragma unroll
+  for (int ti = 0; ti < WM; ++ti) {
+    double p[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
+#pragma unroll
+    for (int tj = 0; tj < WN; ++tj) {
+      const int c = c0 + wc * 64 + tj * 16 + lr;
+      if (tj < nj) {
+        const double gg1 = g1[c], gg2 = g2[c];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = wr * 128 + ti * 16 + 4 * lg + r;
+          const size_t o = (size_t)(i0 + m) * g.npd + c;
+          const double x = g.Xf[o], ft = g.Ftf[o];
+          const double q = (double)acc[ti][tj][r];
+          p[r][0] = fma(x, q, p[r][0]);
+          p[r][1] = fma(ft, q, p[r][1]);
+          p[r][2] = fma(x, gg1, p[r][2]);
+          p[r][3] = fma(ft, gg2, p[r][3]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double v = p[r][q];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+        if (lr == 0) red[wc][wr * 128 + ti * 16 + 4 * lg + r][q] = v;
+      }
+  }
+  __syncthreads();
+  double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np + i0;
+  for (int idx = threadIdx.x; idx < TM * 4; idx += blockDim.x) {
+    const int q = idx / TM, m = idx - q * TM;
+    Pt[(size_t)q * g.Np + m] = red[0][m][q] + red[1][m][q];
+  }
+}
+
+// W[s] = (c1, c2, e1, e2) from the column-tile row sums, tiles added in increasing order.
+__global__ void k_weights_bf16(const double *__restrict__ P, const double *__restrict__ rho,
+                               double *__restrict__ W, int Np, int n_ct, int sb, double kf,
+                               double kr, double ky) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sl = blockIdx.y;
+  if (i >= Np) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  const double *Ps = P + (size_t)sl * n_ct * 4 * Np + i;
+  for (int ct = 0; ct < n_ct; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] += Ps[((size_t)ct * 4 + q) * Np];
+  const double r = rho[i];
+  double *Ws = W + (size_t)sl * 4 * Np;
+  Ws[i] = r * (ky * s[0] / 8 - kr * s[2] / 3);
+  Ws[(size_t)Np + i] = r * (ky * s[1] / 4 - kr * s[3] / 3);
+  Ws[2 * (size_t)Np + i] = -kf * r * s[0] / 6;
+  Ws[3 * (size_t)Np + i] = -kf * r * s[1] / 3;
+}
+
+// ------------------------------------------------------------------------------ GEMM2
+struct Gemm2Args {
+  const uint16_t *Ah, *Al;    // X~^T planes [npd x Np]
+  const float *YTf, *XtTf;    // [npd x Np]
+  const double *W;            // [s_batch x 4 x Np]
+  double *B;                  // [s_batch x npd x npd]
+  int Np, npd, sb, n_mt, n_ct;
+};
+
+// G^T slice (rows c0 + t/4 + 64 h, k0 + 8 (t % 4) ..): formed in FP32 and split.
+__device__ inline void load_g(const Gemm2Args &g, const double *c1, const double *c2, int c0,
+                              int k0, float4 (&ry)[4], float4 (&rx)[4]) {
+  const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r = c0 + row + 64 * h;
+    if (r < g.npd) {
+      const float4 *py = reinterpret_cast<const float4 *>(g.YTf + (size_t)r * g.Np + k0 + kc);
+      const float4 *px = reinterpret_cast<const float4 *>(g.XtTf + (size_t)r * g.Np + k0 + kc);
+      ry[2 * h] = py[0];
+      ry[2 * h + 1] = py[1];
+      rx[2 * h] = px[0];
+      rx[2 * h + 1] = px[1];
+    } else {
+      ry[2 * h] = ry[2 * h + 1] = rx[2 * h] = rx[2 * h + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  }
+}
+
+__device__ inline void store_g(uint16_t *lh, uint16_t *ll, const float (&a)[8], const float (&b)[8],
+                               const float4 (&ry)[4], const float4 (&rx)[4]) {
+  const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const float y[8] = {ry[2 * h].x, ry[2 * h].y, ry[2 * h].z, ry[2 * h].w,
+                        ry[2 * h + 1].x, ry[2 * h + 1].y, ry[2 * h + 1].z, ry[2 * h + 1].w};
+    const float x[8] = {rx[2 * h].x, rx[2 * h].y, rx[2 * h].z, rx[2 * h].w,
+                        rx[2 * h + 1].x, rx[2 * h + 1].y, rx[2 * h + 1].z, rx[2 * h + 1].w};
+    uint32_t hi[4], lo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      split2(fmaf(a[2 * j], y[2 * j], b[2 * j] * x[2 * j]),
+             fmaf(a[2 * j + 1], y[2 * j + 1], b[2 * j + 1] * x[2 * j + 1]), hi[j], lo[j]);
+    *reinterpret_cast<uint4 *>(lh + (row + 64 * h) * LDK + kc) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    *reinterpret_cast<uint4 *>(ll + (row + 64 * h) * LDK + kc) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+  }
+}
+
+__device__ inline void load_w(const double *c1, const double *c2, int k0, float (&a)[8],
+                              float (&b)[8]) {
+  const int kc = (threadIdx.x & 3) * 8;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    a[j] = (float)c1[k0 + kc + j];
+    b[j] = (float)(0.5 * c2[k0 + kc + j]);
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void k_gemm2_bf16x3(Gemm2Args g) {
+  constexpr int TM = TM2, WM = TM2 / 32;
+  __shared__ Stage<TM2> sm;
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id / (g.n_mt * g.n_ct);
+  const int rest = id - sl * g.n_mt * g.n_ct;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int m0 = mt * TM, c0 = ct * TN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1, lr = l & 15, lg = l >> 4;
+  const int mi = clamp_blocks(g.npd, m0 + wr * (TM / 2), WM);
+  const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
+  const double *c1 = g.W + (size_t)sl * 4 * g.Np, *c2 = c1 + g.Np;
+
+  v4f acc[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int KT = g.Np / TK;
+  uint4 rah[TM / 64], ral[TM / 64];
+  float4 ry[4], rx[4];
+  float wa[8], wb[8];
+  load_rows<TM>(g.Ah, g.Np, g.npd, m0, 0, rah);
+  load_rows<TM>(g.Al, g.Np, g.npd, m0, 0, ral);
+  load_g(g, c1, c2, c0, 0, ry, rx);
+  load_w(c1, c2, 0, wa, wb);
+  for (int kt = 0; kt < KT; ++kt) {
+    __syncthreads();
+    store_rows<TM>(sm.ah, rah);
+    store_rows<TM>(sm.al, ral);
+    store_g(sm.bh, sm.bl, wa, wb, ry, rx);
+    __syncthreads();
+    if (kt + 1 < KT) {
+      const int k0 = (kt + 1) * TK;
+      load_rows<TM>(g.Ah, g.Np, g.npd, m0, k0, rah);
+      load_rows<TM>(g.Al, g.Np, g.npd, m0, k0, ral);
+      load_g(g, c1, c2, c0, k0, ry, rx);
+      load_w(c1, c2, k0, wa, wb);
+    }
+    mma_slice<TM>(sm, wr, wc, mi, nj, acc);
+  }
+  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
+#pragma unroll
+  for (int ti = 0; ti < WM; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < WN; ++tj)
+      if (ti < mi && tj < nj) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wr * (TM / 2) + ti * 16 + 4 * lg + r;
+          const int c = c0 + wc * 64 + tj * 16 + lr;
+          Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
+        }
+      }
+}
+
+}  // namespace
+
+int bf16_col_tiles(const Ensemble &e) { return (e.npd + TN - 1) / TN; }
+
+void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
+  Gemm1Args g;
+  const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+  g.Xh = e.X_hi; g.Xl = e.X_lo;
+  g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
+  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_it = e.Np / TM1; g.n_ct = bf16_col_tiles(e);
+  g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  const int total = sb * g.n_it * g.n_ct;
+  hipLaunchKernelGGL(k_gemm1_bf16x3, dim3(8 * ((total + 7) / 8)), dim3(256), 0, st, g);
+  const double kf = (what & kTermF) ? 1.0 : 0.0;
+  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
+  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
+  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
+                     e.W, e.Np, g.n_ct, sb, kf, kr, ky);
+}
+
+void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
+  Gemm2Args g;
+  g.Ah = e.XtT_hi; g.Al = e.XtT_lo;
+  g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.W = e.W; g.B = e.B;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_mt = (e.npd + TM2 - 1) / TM2; g.n_ct = (e.npd + TN - 1) / TN;
+  const int total = sb * g.n_mt * g.n_ct;
+  hipLaunchKernelGGL(k_gemm2_bf16x3, dim3(8 * ((total + 7) / 8)), dim3(256), 0, st, g);
+}
+
+}  // namespace tdscha

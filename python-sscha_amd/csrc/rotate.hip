@@ -97,141 +97,208 @@ __global__ void k_rot_left(const double *__restrict__ in, double *__restrict__ o
   }
 }
 
-// ---- Fused two-sided rotations on (row block A) x (column group) tiles --------------
-// A column group is a run of consecutive whole blocks at most kRotTile columns wide, so
-// both one-sided products of a tile stay inside LDS. Used when every block has at most
-// kRotTile modes.
+// ---- Group-tiled rotations on FP64 MFMA ------------------------------------------------
+// Used when every block has at most kRotTile modes (else the one-sided kernels above run).
+// Modes are cut into groups of whole degenerate blocks, each at most G wide. On the tile
+// (row group g, column group h) the rotations only involve the two groups' diagonal
+// blocks, staged in LDS as dense G x G block-diagonal matrices:
+//   rot1:  Ys_s[g][h]  = S_g^T Ysym[g][h] S_h
+//   rot2:  d2v[g][h]  += S_g (B_s + B_s^T)[g][h] S_h^T
+// Both products are symmetric in (g, h), so only tiles g <= h are computed and each is
+// written to its mirror position too. Products run as v_mfma_f64_16x16x4 over 16 x 16
+// sub-tiles, one wave per sub-tile.
 
-// Ys[s][A][grp] = S_A^T Ysym[A][grp] S_grp   (written as fp64, or as bf16 hi/lo / fp32)
-__global__ __launch_bounds__(256) void k_rot1_fused(const double *__restrict__ Ysym,
-                                                    double *__restrict__ Ys, void *__restrict__ lp,
-                                                    int lp_prec, size_t lp_plane,
-                                                    const double *__restrict__ S,
-                                                    const int4 *__restrict__ blocks,
-                                                    const int2 *__restrict__ groups,
-                                                    const ModeMeta *__restrict__ meta, int npd,
-                                                    int sym_stride, Pass ps) {
-  __shared__ double M[kRotTile * kRotTile];
-  __shared__ double T[kRotTile * kRotTile];
-  const int4 A = blocks[blockIdx.x];
-  const int2 grp = groups[blockIdx.y];
-  const int sl = blockIdx.z, s = ps.sym(sl);
-  Ysym += (size_t)ps.chain(sl) * npd * npd;
-  const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
-  const double *Ss = S + (size_t)s * sym_stride;
-  const double *SA = Ss + A.z;
-  const int cnt = dA * W;
-  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-    const int r = idx / W, c = idx - r * W;
-    M[r * kRotTile + c] = Ysym[(size_t)(offA + r) * npd + g0 + c];
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int G> struct GrpTile {
+  static constexpr int LD = G + 2;               // LDS row stride in doubles
+  static constexpr int TPW = (G / 16) * (G / 16) / 4;   // 16 x 16 sub-tiles per wave
+};
+
+// Dense S_g of symmetry Ss (zero outside g's blocks and outside the group width).
+template <int G>
+__device__ inline void build_group_s(double *Sl, const double *__restrict__ Ss, int2 grp,
+                                     const ModeMeta *__restrict__ meta) {
+  constexpr int LD = GrpTile<G>::LD;
+  for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
+    const int r = idx / G, c = idx - r * G;
+    double v = 0.0;
+    if (r < grp.y && c < grp.y) {
+      const ModeMeta mr = meta[grp.x + r];
+      const int lb = mr.blk_off - grp.x, cc = c - lb;
+      if (cc >= 0 && cc < mr.blk_dim)
+        v = Ss[mr.sym_off + (grp.x + r - mr.blk_off) * mr.blk_dim + cc];
+    }
+    Sl[r * LD + c] = v;
   }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-    const int r = idx / W, c = idx - r * W;
-    const ModeMeta mb = meta[g0 + c];
-    const int cb = mb.blk_off - g0, d = mb.blk_dim, cp = g0 + c - mb.blk_off;
-    const double *SB = Ss + mb.sym_off;
-    double acc = 0.0;
-    for (int l = 0; l < d; ++l) acc = fma(M[r * kRotTile + cb + l], SB[l * d + cp], acc);
-    T[r * kRotTile + c] = acc;
-  }
-  __syncthreads();
-  const size_t base = (size_t)sl * npd * npd;
-  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-    const int r = idx / W, c = idx - r * W;
-    double acc = 0.0;
-    for (int k = 0; k < dA; ++k) acc = fma(SA[k * dA + r], T[k * kRotTile + c], acc);
-    const size_t o = base + (size_t)(offA + r) * npd + g0 + c;
-    if (Ys) Ys[o] = acc;
-    if (lp_prec == kBF16x3) {
-      uint16_t *hi = (uint16_t *)lp;
-      const float f = (float)acc;
-      const uint16_t h = bf16_rne(f);
-      hi[o] = h;
-      hi[lp_plane + o] = bf16_rne(f - __uint_as_float((uint32_t)h << 16));
-    } else if (lp_prec == kFP32) {
-      ((float *)lp)[o] = (float)acc;
+}
+
+// acc[q] (sub-tile q of this wave) = sum_k A(i, k) B(k, j) with
+//   A(i, k) = TA ? Am[k][i] : Am[i][k],   B(k, j) = TB ? Bm[j][k] : Bm[k][j]
+// over k < G; the result element r of a lane is row lg + 4 r, column lr of the sub-tile.
+template <int G, bool TA, bool TB>
+__device__ inline void grp_mma(const double *Am, const double *Bm, v4d (&acc)[GrpTile<G>::TPW]) {
+  constexpr int LD = GrpTile<G>::LD, NT = G / 16;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+#pragma unroll
+  for (int q = 0; q < GrpTile<G>::TPW; ++q) {
+    const int t = w + 4 * q, ti = t / NT, tj = t - ti * NT;
+    const int i = ti * 16 + lr, j = tj * 16 + lr;
+#pragma unroll
+    for (int kq = 0; kq < G / 4; ++kq) {
+      const int k = kq * 4 + lg;
+      const double a = TA ? Am[k * LD + i] : Am[i * LD + k];
+      const double b = TB ? Bm[j * LD + k] : Bm[k * LD + j];
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
     }
   }
 }
 
-// d2v[A][grp] += sum_s S_A (B_s + B_s^T)[A][grp] S_grp^T   (one workgroup owns its tile)
-__global__ __launch_bounds__(256) void k_rot2_fused(const double *__restrict__ B,
-                                                    double *__restrict__ d2v,
-                                                    const double *__restrict__ S,
-                                                    const int4 *__restrict__ blocks,
-                                                    const int2 *__restrict__ groups,
-                                                    const ModeMeta *__restrict__ meta, int npd,
-                                                    int sym_stride, Pass ps) {
-  __shared__ double M[kRotTile * kRotTile];
-  __shared__ double T[kRotTile * kRotTile];
-  constexpr int kPer = kRotTile * kRotTile / 256;
-  const int4 A = blocks[blockIdx.x];
-  const int2 grp = groups[blockIdx.y];
-  const int ch = blockIdx.z;
-  d2v += (size_t)ch * npd * npd;
-  const int offA = A.x, dA = A.y, g0 = grp.x, W = grp.y;
-  const int cnt = dA * W;
-  // the images are added one at a time onto the running sum, so the result does not depend
-  // on how the symmetries are split into passes
-  double acc[kPer];
+template <int G>
+__device__ inline void grp_store_lds(double *Tl, const v4d (&acc)[GrpTile<G>::TPW]) {
+  constexpr int LD = GrpTile<G>::LD, NT = G / 16;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int idx = threadIdx.x + 256 * j;
-    acc[j] = 0.0;
-    if (idx < cnt) {
-      const int r = idx / W, c = idx - r * W;
-      acc[j] = d2v[(size_t)(offA + r) * npd + g0 + c];
+  for (int q = 0; q < GrpTile<G>::TPW; ++q) {
+    const int t = w + 4 * q, ti = t / NT, tj = t - ti * NT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Tl[(ti * 16 + lg + 4 * r) * LD + tj * 16 + lr] = acc[q][r];
+  }
+}
+
+// One output element of the rotated tile in the requested storage.
+__device__ inline void store_ys(double *Ys, void *lp, int lp_prec, size_t lp_plane, size_t o,
+                                double v) {
+  if (Ys) Ys[o] = v;
+  if (lp_prec == kBF16x3) {
+    uint16_t *hi = (uint16_t *)lp;
+    const float f = (float)v;
+    const uint16_t h = bf16_rne(f);
+    hi[o] = h;
+    hi[lp_plane + o] = bf16_rne(f - bf16_to_f(h));
+  } else if (lp_prec == kFP32) {
+    ((float *)lp)[o] = (float)v;
+  }
+}
+
+// grid (tile pairs g <= h, slot splits, chains); the workgroup rotates the slots
+// t = blockIdx.y, blockIdx.y + gridDim.y, ... of its chain.
+template <int G>
+__global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysym,
+                                                  double *__restrict__ Ys, void *__restrict__ lp,
+                                                  int lp_prec, size_t lp_plane,
+                                                  const double *__restrict__ S,
+                                                  const int2 *__restrict__ groups,
+                                                  const int2 *__restrict__ pairs,
+                                                  const ModeMeta *__restrict__ meta, int npd,
+                                                  int sym_stride, Pass ps) {
+  constexpr int LD = GrpTile<G>::LD, TPW = GrpTile<G>::TPW, NT = G / 16;
+  __shared__ double M[G * LD], T[G * LD], Sg[G * LD], Sh[G * LD];
+  const int2 pr = pairs[blockIdx.x];
+  const int2 gg = groups[pr.x], gh = groups[pr.y];
+  const int ch = blockIdx.z;
+  const double *Yc = Ysym + (size_t)ch * npd * npd;
+  for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
+    const int r = idx / G, c = idx - r * G;
+    M[r * LD + c] = (r < gg.y && c < gh.y) ? Yc[(size_t)(gg.x + r) * npd + gh.x + c] : 0.0;
+  }
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  for (int t = blockIdx.y; t < ps.ns; t += gridDim.y) {
+    const int sl = ch * ps.ns + t;
+    const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
+    __syncthreads();   // previous slot's reads of Sg / Sh / T are done
+    build_group_s<G>(Sg, Ss, gg, meta);
+    build_group_s<G>(Sh, Ss, gh, meta);
+    __syncthreads();
+    v4d acc[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) acc[q] = (v4d){0.0, 0.0, 0.0, 0.0};
+    grp_mma<G, false, false>(M, Sh, acc);           // T = M S_h
+    grp_store_lds<G>(T, acc);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) acc[q] = (v4d){0.0, 0.0, 0.0, 0.0};
+    grp_mma<G, true, false>(Sg, T, acc);            // Ys = S_g^T T
+    const size_t base = (size_t)sl * npd * npd;
+#pragma unroll
+    for (int q = 0; q < TPW; ++q) {
+      const int tt = w + 4 * q, ti = tt / NT, tj = tt - ti * NT;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + lg + 4 * r, j = tj * 16 + lr;
+        if (i < gg.y && j < gh.y) {
+          store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gg.x + i) * npd + gh.x + j,
+                   acc[q][r]);
+          if (pr.x != pr.y)
+            store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gh.x + j) * npd + gg.x + i,
+                     acc[q][r]);
+        }
+      }
+    }
+  }
+}
+
+// grid (tile pairs g <= h, chains). The running sum of the tile is held in the MFMA
+// accumulators; every image of the pass is added onto it in symmetry order, so the result
+// does not depend on how the symmetries are split into passes.
+template <int G>
+__global__ __launch_bounds__(256) void k_rot2_grp(const double *__restrict__ B,
+                                                  double *__restrict__ d2v,
+                                                  const double *__restrict__ S,
+                                                  const int2 *__restrict__ groups,
+                                                  const int2 *__restrict__ pairs,
+                                                  const ModeMeta *__restrict__ meta, int npd,
+                                                  int sym_stride, Pass ps) {
+  constexpr int LD = GrpTile<G>::LD, TPW = GrpTile<G>::TPW, NT = G / 16;
+  __shared__ double M[G * LD], T[G * LD], Sg[G * LD], Sh[G * LD];
+  const int2 pr = pairs[blockIdx.x];
+  const int2 gg = groups[pr.x], gh = groups[pr.y];
+  const int ch = blockIdx.y;
+  d2v += (size_t)ch * npd * npd;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  v4d acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int tt = w + 4 * q, ti = tt / NT, tj = tt - ti * NT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = ti * 16 + lg + 4 * r, j = tj * 16 + lr;
+      acc[q][r] = (i < gg.y && j < gh.y) ? d2v[(size_t)(gg.x + i) * npd + gh.x + j] : 0.0;
     }
   }
   for (int t = 0; t < ps.ns; ++t) {
     const int sl = ch * ps.ns + t;
     const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
-    const double *SA = Ss + A.z;
     const double *Bs = B + (size_t)sl * npd * npd;
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-      const int r = idx / W, c = idx - r * W;
-      M[r * kRotTile + c] = Bs[(size_t)(offA + r) * npd + g0 + c];
+    for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
+      const int r = idx / G, c = idx - r * G;
+      double v = 0.0;
+      if (r < gg.y && c < gh.y)
+        v = Bs[(size_t)(gg.x + r) * npd + gh.x + c] + Bs[(size_t)(gh.x + c) * npd + gg.x + r];
+      M[r * LD + c] = v;
     }
-    // the transposed part, read along rows of B (r fastest) and staged in T
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-      const int c = idx / dA, r = idx - c * dA;
-      T[c * kRotTile + r] = Bs[(size_t)(g0 + c) * npd + offA + r];
-    }
+    build_group_s<G>(Sg, Ss, gg, meta);
+    build_group_s<G>(Sh, Ss, gh, meta);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-      const int r = idx / W, c = idx - r * W;
-      M[r * kRotTile + c] += T[c * kRotTile + r];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
-      const int r = idx / W, c = idx - r * W;
-      const ModeMeta mb = meta[g0 + c];
-      const int cb = mb.blk_off - g0, d = mb.blk_dim, cp = g0 + c - mb.blk_off;
-      const double *SB = Ss + mb.sym_off + cp * d;
-      double t = 0.0;
-      for (int l = 0; l < d; ++l) t = fma(M[r * kRotTile + cb + l], SB[l], t);
-      T[r * kRotTile + c] = t;
-    }
-    __syncthreads();
+    v4d tq[TPW];
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      const int idx = threadIdx.x + 256 * j;
-      if (idx < cnt) {
-        const int r = idx / W, c = idx - r * W;
-        double t = 0.0;
-        for (int k = 0; k < dA; ++k) t = fma(SA[r * dA + k], T[k * kRotTile + c], t);
-        acc[j] += t;
-      }
-    }
+    for (int q = 0; q < TPW; ++q) tq[q] = (v4d){0.0, 0.0, 0.0, 0.0};
+    grp_mma<G, false, true>(M, Sh, tq);             // T = (B + B^T) S_h^T
+    grp_store_lds<G>(T, tq);
+    __syncthreads();
+    grp_mma<G, false, false>(Sg, T, acc);           // acc += S_g T
   }
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int idx = threadIdx.x + 256 * j;
-    if (idx < cnt) {
-      const int r = idx / W, c = idx - r * W;
-      d2v[(size_t)(offA + r) * npd + g0 + c] = acc[j];
+  for (int q = 0; q < TPW; ++q) {
+    const int tt = w + 4 * q, ti = tt / NT, tj = tt - ti * NT;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = ti * 16 + lg + 4 * r, j = tj * 16 + lr;
+      if (i < gg.y && j < gh.y) {
+        d2v[(size_t)(gg.x + i) * npd + gh.x + j] = acc[q][r];
+        if (pr.x != pr.y) d2v[(size_t)(gh.x + j) * npd + gg.x + i] = acc[q][r];
+      }
     }
   }
 }
@@ -350,16 +417,29 @@ void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pas
 
 void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st) {
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
-  dim3 grid(e.n_blocks, e.n_groups, ps.slots());
-  hipLaunchKernelGGL(k_rot1_fused, grid, dim3(256), 0, st, e.Ysym,
-                     prec == kFP64 ? e.Ys : (double *)nullptr, e.ys_lp(prec), prec, plane, e.S,
-                     e.dblocks, e.dgroups, e.meta, e.npd, e.sym_stride, ps);
+  // enough workgroups to fill the chip: split each chain's slots over gridDim.y
+  const long tiles = (long)e.n_pairs * ps.nc;
+  int split = (int)((2048 + tiles - 1) / tiles);
+  if (split > ps.ns) split = ps.ns;
+  if (split < 1) split = 1;
+  dim3 grid(e.n_pairs, split, ps.nc);
+  double *ys = prec == kFP64 ? e.Ys : (double *)nullptr;
+  if (e.grp_width == 32)
+    hipLaunchKernelGGL(k_rot1_grp<32>, grid, dim3(256), 0, st, e.Ysym, ys, e.ys_lp(prec), prec,
+                       plane, e.S, e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+  else
+    hipLaunchKernelGGL(k_rot1_grp<64>, grid, dim3(256), 0, st, e.Ysym, ys, e.ys_lp(prec), prec,
+                       plane, e.S, e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
 }
 
 void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st) {
-  dim3 grid(e.n_blocks, e.n_groups, ps.nc);
-  hipLaunchKernelGGL(k_rot2_fused, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dblocks,
-                     e.dgroups, e.meta, e.npd, e.sym_stride, ps);
+  dim3 grid(e.n_pairs, ps.nc);
+  if (e.grp_width == 32)
+    hipLaunchKernelGGL(k_rot2_grp<32>, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dgroups,
+                       e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+  else
+    hipLaunchKernelGGL(k_rot2_grp<64>, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dgroups,
+                       e.dpairs, e.meta, e.npd, e.sym_stride, ps);
 }
 
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {

@@ -107,14 +107,24 @@ int check_blocks(int n, int n_blocks, const int *N_deg, const int *const *deg_sp
 
 int choose_s_batch(const Ensemble &e, int requested) {
   if (requested > 0) return requested < e.N_sym ? requested : e.N_sym;
-  const long per1 = e.Np / kPadRows;
-  const long per2 = (long)(e.npd / kPadCols) * (e.npd / kPadCols);
-  long per = per1 < per2 ? per1 : per2;
+  // workgroups per symmetry image of the least parallel contraction (GEMM2: n x n output
+  // tiles), aiming at about 4096 workgroups per launch
+  long per;
+  if (e.prec_mask & (1 << kBF16x3)) {
+    per = (long)((e.npd + 255) / 256) * ((e.npd + kBfTileN - 1) / kBfTileN);
+  } else {
+    const long per1 = e.Np / 128;
+    const long per2 = (long)(e.npd / kPadCols) * (e.npd / kPadCols);
+    per = per1 < per2 ? per1 : per2;
+  }
+  if (per < 1) per = 1;
   long sb = (4096 + per - 1) / per;
-  // workspace per symmetry: Tw, Ys, B (npd^2 doubles each) + Ys_lp + W + G + V
+  // workspace per symmetry: Tw, Ys, B (npd^2 doubles each) + reduced-precision Ys + W + G + V
+  // + the BF16x3 row sums
   const double per_s_bytes = 3.0 * 8.0 * e.npd * (double)e.npd + 4.0 * e.npd * (double)e.npd +
-                             8.0 * (4.0 * e.Np + 3.0 * e.npd);
-  const double budget = 6.0e9;
+                             8.0 * (4.0 * e.Np + 3.0 * e.npd) +
+                             8.0 * 4.0 * e.Np * ((e.npd + kBfTileN - 1) / kBfTileN);
+  const double budget = 12.0e9;
   long cap = (long)(budget / per_s_bytes);
   if (cap < 1) cap = 1;
   if (sb > cap) sb = cap;
@@ -226,6 +236,9 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   rc |= dev_alloc(h, &e.B, sb * nn);
   rc |= dev_alloc(h, &e.V, sb * e.npd);
   rc |= dev_alloc(h, &e.Vp, (size_t)((e.Np + kFavChunk - 1) / kFavChunk) * sb * e.npd);
+  e.Pw = nullptr;
+  if (e.prec_mask & (1 << kBF16x3))
+    rc |= dev_alloc(h, &e.Pw, sb * bf16_col_tiles(e) * 4 * (size_t)e.Np);
   e.Ys_f32 = nullptr;
   e.Ys_bf = nullptr;
   if (e.prec_mask & (1 << kFP32)) rc |= dev_alloc(h, &e.Ys_f32, sb * nn);
@@ -398,20 +411,22 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= dev_alloc(h, &e.nocc, (size_t)n);
   rc |= dev_alloc(h, &e.S, (size_t)N_sym * e.sym_stride);
   rc |= dev_alloc(h, &e.meta, (size_t)npd);
-  // fused rotations: every block fits a tile; column groups = runs of whole blocks
-  std::vector<int4> hblocks(n_blocks);
-  std::vector<int2> hgroups;
+  // group-tiled rotations: every block fits a tile; groups = runs of whole blocks
+  std::vector<int2> hgroups, hpairs;
   e.fused_rot = e.max_block <= kRotTile;
+  e.grp_width = e.max_block <= 32 ? 32 : 64;
   for (int b = 0; b < n_blocks; ++b) {
-    hblocks[b] = make_int4(e.blk_off[b], e.blk_dim[b], e.sym_off[b], 0);
-    if (hgroups.empty() || hgroups.back().y + e.blk_dim[b] > kRotTile)
+    if (hgroups.empty() || hgroups.back().y + e.blk_dim[b] > e.grp_width)
       hgroups.push_back(make_int2(e.blk_off[b], e.blk_dim[b]));
     else
       hgroups.back().y += e.blk_dim[b];
   }
   e.n_groups = (int)hgroups.size();
-  rc |= dev_alloc(h, &e.dblocks, (size_t)n_blocks);
+  for (int g = 0; g < e.n_groups; ++g)
+    for (int k = g; k < e.n_groups; ++k) hpairs.push_back(make_int2(g, k));
+  e.n_pairs = (int)hpairs.size();
   rc |= dev_alloc(h, &e.dgroups, hgroups.size());
+  rc |= dev_alloc(h, &e.dpairs, hpairs.size());
   if (!rc) rc |= alloc_slots(h, e.s_batch);
   if (!rc) rc |= ensure_chains(h, 1);
   const bool want32 = e.prec_mask & (1 << kFP32), wantbf = e.prec_mask & (1 << kBF16x3);
@@ -420,7 +435,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
     rc |= dev_alloc(h, &e.XtT_f, NX);
     rc |= dev_alloc(h, &e.YT_f, NX);
   }
-  if (want32) rc |= dev_alloc(h, &e.X_f, NX);
+  if (want32 || wantbf) rc |= dev_alloc(h, &e.X_f, NX);
+  if (wantbf) rc |= dev_alloc(h, &e.Ft_f, NX);
   if (wantbf) {
     rc |= dev_alloc(h, &e.X_hi, 2 * NX);
     rc |= dev_alloc(h, &e.XtT_hi, 2 * NX);
@@ -447,8 +463,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   rc |= up(e.w, w, (size_t)n * 8);
   rc |= up(e.nocc, nocc.data(), nocc.size() * 8);
   rc |= up(e.meta, meta.data(), meta.size() * sizeof(ModeMeta));
-  rc |= up(e.dblocks, hblocks.data(), hblocks.size() * sizeof(int4));
   rc |= up(e.dgroups, hgroups.data(), hgroups.size() * sizeof(int2));
+  rc |= up(e.dpairs, hpairs.data(), hpairs.size() * sizeof(int2));
   for (int b = 0; b < n_blocks && !rc; ++b) {
     const size_t d2 = (size_t)e.blk_dim[b] * e.blk_dim[b];
     hipError_t err = hipMemcpy2D(e.S + e.sym_off[b], (size_t)e.sym_stride * 8, symmetries[b], d2 * 8,
@@ -464,7 +480,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
     launch_convert(e.XtT, e.XtT_f, NX, kFP32, nullptr);
     launch_convert(e.YT, e.YT_f, NX, kFP32, nullptr);
   }
-  if (want32) launch_convert(e.X, e.X_f, NX, kFP32, nullptr);
+  if (want32 || wantbf) launch_convert(e.X, e.X_f, NX, kFP32, nullptr);
+  if (wantbf) launch_convert(e.Ft, e.Ft_f, NX, kFP32, nullptr);
   if (wantbf) {
     launch_convert(e.X, e.X_hi, NX, kBF16x3, nullptr);
     launch_convert(e.XtT, e.XtT_hi, NX, kBF16x3, nullptr);

@@ -22,12 +22,13 @@
 
 namespace tdscha {
 
-constexpr int kPadRows = 128;  // config padding (GEMM1 row tile, GEMM2 k tile)
+constexpr int kPadRows = 256;  // config padding (GEMM1 row tiles, GEMM2 k slices)
 constexpr int kPadCols = 64;   // mode padding (GEMM tiles along modes)
 constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 
 constexpr int kRotTile = 64;   // widest block / column group of the fused rotations
 constexpr int kFavChunk = 1024; // configurations per split-k chunk of the f_av product
+constexpr int kBfTileN = 128;   // column tile of the BF16x3 contractions
 
 enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
 
@@ -75,14 +76,16 @@ struct Ensemble {
   double *fups = nullptr, *fpsi = nullptr, *S = nullptr;
   double *w = nullptr, *nocc = nullptr;   // frequencies [n] and Bose occupations [n]
   ModeMeta *meta = nullptr;
-  // fused rotations: row blocks (offset, size, sym offset, 0) and column groups (start, width)
+  // group-tiled rotations (every block at most kRotTile wide): groups of whole blocks
+  // (start, width <= grp_width) and the tile pairs (g, h), g <= h
   bool fused_rot = false;
-  int n_groups = 0;
-  int4 *dblocks = nullptr;
+  int grp_width = 64;
+  int n_groups = 0, n_pairs = 0;
   int2 *dgroups = nullptr;
+  int2 *dpairs = nullptr;
   // reduced-precision operand copies (bf16 stored as raw 16-bit words; lo = hi + count)
   uint16_t *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
-  float *X_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
+  float *X_f = nullptr, *Ft_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
 
   // per-chain inputs / accumulators, for up to n_chains chains (grown on demand)
   int n_chains = 0;
@@ -105,6 +108,7 @@ struct Ensemble {
   double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
   double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
+  double *Pw = nullptr;       // [s_batch x col tiles x 4 x Np]  BF16x3 GEMM1 row sums
 };
 
 void set_error(const std::string &msg);
@@ -134,6 +138,10 @@ void launch_post(const Ensemble &e, int chain, const double *psi, double *out, i
                  double shift, hipStream_t st);
 
 // ---- GEMMs (gemm.hip) ----
+// BF16x3 forms (gemm_bf16.hip)
+int bf16_col_tiles(const Ensemble &e);
+void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st);
+void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st);
 // V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 // GEMM1: per image (config i, symmetry s) the quadratic forms

@@ -79,6 +79,18 @@ def test_blocks_wider_than_rotation_tile(precision):
     _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("max_block,seed", [(32, 40), (50, 41)])
+def test_group_tiled_rotation_widths(precision, max_block, seed):
+    """Blocks of up to 32 modes take the 32-wide group tiles, up to 64 the 64-wide ones."""
+    ens = fixtures.random_ensemble(seed, 170, 70, 4, max_block=max_block)
+    assert 16 < max(ens.sym.N_degeneracy) <= max_block
+    R1, Y1 = fixtures.random_perturbation(seed + 5, 170)
+    tol = {"fp64": TOL64, "fp32": 2e-5, "bf16x3": 2e-4}[precision]
+    dev = _dev_ens(ens, precisions=("fp64", precision))
+    _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
 def test_precision_forms(precision, tol):
     """FP32 / BF16x3 contractions against the FP64 oracle (relative to the abs-sum scale)."""
