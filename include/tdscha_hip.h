@@ -164,6 +164,19 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
  * that many chains would do), so that later calls allocate nothing. */
 int tdscha_ensemble_reserve(tdscha_ensemble *ens, int n_chains);
 
+/* ---- Force adjoint (no counterpart in the reference: the derivative of the
+ * anharmonic part of apply_full_L, Py:3467-3570 with apply_anharmonic_FT Py:2891-3281, with
+ * respect to the configuration forces Y) --------------------------------------------------
+ * For pairs p = 0..n_pairs-1 of device vectors (u_p, v_p) of length n + n (n + 1):
+ *   grad (+)= d/dY  sum_p  u_p . A_p(v_p; Y)
+ * where A_p is the anharmonic part of the operator of tdscha_apply_full_L with flags[p]
+ * (TRANSPOSE / WIGNER / REVERSE / IGNORE_V3 / IGNORE_V4 as there; IGNORE_V4 equal for all
+ * pairs; the harmonic part and the shift do not depend on Y). grad: device, N x n
+ * row-major FP64 (the layout of Y); written (accumulate = 0) or added to. FP64 throughout. */
+int tdscha_ensemble_force_grad(tdscha_ensemble *ens, int n_pairs, const int *flags,
+                               const double *const *u, const double *const *v, double *grad,
+                               int accumulate, void *stream);
+
 /* ---- Device-resident Lanczos (the recurrence of Lanczos.run_FT, Py:5843-6372) ---------
  * n_chains starting vectors advance together: every step is one batched L application
  * (L q and L^T p of every chain, or L q only for the symmetric Wigner form) followed by the

@@ -22,6 +22,11 @@ Contents
   - Wigner dot-product mask                 Py:5790-5839
   - biconjugate / symmetric Lanczos run_FT  Py:5843-6372
   - continued-fraction Green function       Py:4951-5047
+* The derivative of a linear functional of the kernel outputs with respect to the
+  configuration forces Y (``force_grad``), image by image in the reference's own form
+  (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
+  (``post_seeds``). Pinned by finite differences of ``perturb_average_sym`` and of
+  ``apply_anharmonic_FT``.
 """
 from __future__ import annotations
 
@@ -414,6 +419,75 @@ def anharmonic_post(f_av, d2v, w, T, transpose=False, use_wigner=False):
         sign = -1.0
     out = np.concatenate([f_av, pack_upper(pert_Y), pack_upper(pert_RA)])
     return sign * out
+
+
+def f_psi(w, T):
+    """C:24-30: (1 + 2n) / (2w)."""
+    return (1 + 2 * bose_occupation(w, T)) / (2 * np.asarray(w))
+
+
+def force_grad(X, Y, w, rho, R1, Y1, T, sym: SymBlocks, uR, UD, kf=1.0, kr=1.0, ky=1.0,
+               norm=None):
+    """dJ/dY (N x n) for J = uR . f_av + <UD, d2v>.
+
+    f_av, d2v are the outputs of perturb_average_sym with the per-term switches of
+    C:2082-2754 (kf: the f_av terms, kr: the R1 terms of d2v, ky: the Y1 terms of d2v).
+    Per image, with u = S x, f = S y, ut = f_ups u, ft = f_psi f, q = Y1s u:
+      dJ/df = [e1 uR + (-kf rho/3)(uR.ut) f_psi q + 2 c1 U ut
+               + rho (ut^T U ut)(ky/4 f_psi q - kr/3 R1)] / (N_eff N_sym)
+    and dJ/dy = S^T dJ/df; U is the symmetric part of UD (d2v is symmetric).
+    """
+    X, Y, R1 = (np.asarray(a, dtype=np.float64) for a in (X, Y, R1))
+    N, n = X.shape
+    fu, fp = f_ups(w, T), f_psi(w, T)
+    Y1s = 0.5 * (np.asarray(Y1) + np.asarray(Y1).T)
+    U = 0.5 * (np.asarray(UD) + np.asarray(UD).T)
+    uR = np.asarray(uR, dtype=np.float64)
+    rho = np.asarray(rho, dtype=np.float64)
+    neff, nsym = norm if norm is not None else (float(np.sum(rho)), float(sym.n_syms))
+    G = np.zeros((N, n))
+    for s in range(sym.n_syms):
+        S = sym.dense(s)
+        u = X @ S.T
+        ut = u * fu
+        q = u @ Y1s
+        alpha = np.einsum("ia,ia->i", u, q)
+        gamma = ut @ R1
+        e1 = -kf * rho * alpha / 6
+        c1 = rho * (ky * alpha / 8 - kr * gamma / 3)
+        nu = ut @ uR
+        Uut = ut @ U
+        mu = np.einsum("ia,ia->i", ut, Uut)
+        g = (e1[:, None] * uR[None, :] + 2 * c1[:, None] * Uut
+             + (rho * (-kf * nu / 3 + ky * mu / 4))[:, None] * (fp[None, :] * q)
+             - (kr * rho * mu / 3)[:, None] * R1[None, :])
+        G += g @ S
+    return G / (neff * nsym)
+
+
+def post_seeds(u, w, T, transpose=False, use_wigner=False, reverse_L=False):
+    """(uR, UD) with u . anharmonic_post(f, d2v) = uR . f + <UD, d2v> for every f and
+    symmetric d2v, found by applying anharmonic_post to unit inputs (n + n(n+1)/2 calls)."""
+    n = len(w)
+    sign = -1.0 if reverse_L else 1.0
+    uR = np.zeros(n)
+    UD = np.zeros((n, n))
+    zero_d = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        uR[j] = sign * u.dot(anharmonic_post(e, zero_d, w, T, transpose, use_wigner))
+    zero_f = np.zeros(n)
+    for i in range(n):
+        for j in range(i, n):
+            D = np.zeros((n, n))
+            D[i, j] = D[j, i] = 1.0
+            v = sign * u.dot(anharmonic_post(zero_f, D, w, T, transpose, use_wigner))
+            if i == j:
+                UD[i, i] = v
+            else:
+                UD[i, j] = UD[j, i] = 0.5 * v
+    return uR, UD
 
 
 @dataclass

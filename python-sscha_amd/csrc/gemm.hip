@@ -56,6 +56,7 @@ struct Gemm1Args {
   double *W;
   int Np, npd;
   int do_q;
+  int raw;           // write alpha, beta, gamma, delta themselves instead of the weights
   double kf, kr, ky;
 };
 
@@ -211,9 +212,15 @@ __global__ __launch_bounds__(256) void k_gemm1(Gemm1Args<T> g) {
   __syncthreads();
   if (threadIdx.x < BM) {
     const int m = threadIdx.x, i = i0 + m;
-    write_weights(g.W + (size_t)sl * 4 * g.Np, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
-                  red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
-                  red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
+    double *Ws = g.W + (size_t)sl * 4 * g.Np;
+    if (g.raw) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Ws[(size_t)q * g.Np + i] = red[0][m][q] + red[1][m][q];
+    } else {
+      write_weights(Ws, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
+                    red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
+                    red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
+    }
   }
 }
 
@@ -382,6 +389,116 @@ __global__ void k_fav_reduce(const double *__restrict__ Vp, double *__restrict__
   V[i] = acc;
 }
 
+// ---------------------------------------------------------------- force adjoint
+// dJ/dY for J = sum over chains of uR . f_av + <U, d2v> (seeds from k_post_adj), per image
+//   dJ/dy_i = sum_s 2 c1 (U~ x~_i) + rho (-kf nu/3 + ky mu/4) f_psi (Ys x_i)
+//             + e1 S^T uR - kr rho mu/3 S^T R1
+// with U~ = S^T U S, mu = x~^T U~ x~, nu = x~ . S^T uR and the forward weights c1, e1 of
+// GEMM1. One K-concatenated FP64 MFMA product per
+// (configs x modes) tile: for every slot of the pass the two row-scaled products
+// (2 c1 X)(U~ f_ups) and (k X)(Ys f_psi), then the 2 sb rank-1 terms as extra k columns.
+struct AdjArgs {
+  const double *X;              // [Np x npd]
+  const double *Us, *Ys;        // [sb x npd x npd]  f_ups S^T U S f_ups, S^T Ysym S
+  const double *W, *Wu;         // [sb x 4 x Np]     forward weights, (mu, -, nu, -)
+  const double *G, *Gu;         // [sb x 2 x npd]    (f_ups S^T R1, S^T R1 / f_psi), (., S^T uR)
+  const double *rho, *fups, *fpsi;
+  double *grad;                 // [Np x npd]        accumulated
+  int Np, npd, n, sb;
+  double kf, kr, ky;
+};
+
+__device__ inline void adj_load(const AdjArgs &g, int step, int i0, int c0, double (&ra)[4],
+                                double (&rb)[4]) {
+  const int t = threadIdx.x, row = t >> 2, kk = (t & 3) * 4;
+  const int i = i0 + row, c = c0 + row;
+  const int KT = g.npd / BK;
+  const int main_steps = g.sb * 2 * KT;
+  if (step < main_steps) {
+    const int slot = step / (2 * KT), half = (step / KT) & 1, k0 = (step % KT) * BK + kk;
+    const double *Ws = g.W + (size_t)slot * 4 * g.Np, *Wus = g.Wu + (size_t)slot * 4 * g.Np;
+    const double ca = half == 0 ? 2.0 * Ws[i]
+                                : g.rho[i] * (-g.kf * Wus[2 * (size_t)g.Np + i] / 3 +
+                                              g.ky * Wus[i] / 4);
+    const double *xr = g.X + (size_t)i * g.npd + k0;
+    const double *mr = (half == 0 ? g.Us : g.Ys) + (size_t)slot * g.npd * g.npd +
+                       (size_t)c * g.npd + k0;
+    const double cb = half == 0 ? (c < g.n ? 1.0 / g.fups[c] : 0.0) : g.fpsi[c];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ra[j] = ca * xr[j];
+      rb[j] = cb * mr[j];
+    }
+  } else {
+    const int k0 = (step - main_steps) * BK + kk;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kap = k0 + j, slot = kap >> 1;
+      ra[j] = rb[j] = 0.0;
+      if (slot < g.sb) {
+        const double *Ws = g.W + (size_t)slot * 4 * g.Np, *Wus = g.Wu + (size_t)slot * 4 * g.Np;
+        if ((kap & 1) == 0) {
+          ra[j] = Ws[2 * (size_t)g.Np + i];                        // e1
+          rb[j] = g.Gu[(size_t)slot * 2 * g.npd + g.npd + c];      // S^T uR
+        } else {
+          ra[j] = -g.kr * g.rho[i] * Wus[i] / 3;                   // -kr rho mu / 3
+          rb[j] = g.G[(size_t)slot * 2 * g.npd + g.npd + c] * g.fpsi[c];   // S^T R1
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_grad(AdjArgs g) {
+  __shared__ double As[2][BK * LDS_LD];
+  __shared__ double Bs[2][BK * LDS_LD];
+  const int i0 = blockIdx.x * BM, c0 = blockIdx.y * BN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 1, wc = w & 1;
+  const int lr = l & 15, lg = l >> 4;
+  v4d acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+  const int steps = g.sb * 2 * (g.npd / BK) + (2 * g.sb + BK - 1) / BK;
+  double ra[4], rb[4];
+  adj_load(g, 0, i0, c0, ra, rb);
+  store_slice(As[0], ra);
+  store_slice(Bs[0], rb);
+  __syncthreads();
+  for (int st = 0; st < steps; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < steps) adj_load(g, st + 1, i0, c0, ra, rb);
+    mma_slice(As[cur], Bs[cur], wr, wc, acc);
+    if (st + 1 < steps) {
+      store_slice(As[cur ^ 1], ra);
+      store_slice(Bs[cur ^ 1], rb);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = i0 + wr * 32 + ti * 16 + out_row<double>(lg, r);
+        const int c = c0 + wc * 32 + tj * 16 + lr;
+        g.grad[(size_t)m * g.npd + c] += acc[ti][tj][r];
+      }
+}
+
+// grad[i][c] (N x n) = (or +=) gradp[i][c] (Np x npd)
+__global__ void k_grad_out(const double *__restrict__ gp, double *__restrict__ grad, int N, int n,
+                           int npd, int accumulate) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (c >= n || i >= N) return;
+  const double v = gp[(size_t)i * npd + c];
+  grad[(size_t)i * n + c] = accumulate ? grad[(size_t)i * n + c] + v : v;
+}
+
 }  // namespace
 
 void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st) {
@@ -395,15 +512,24 @@ void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st)
     Gemm1Args<float> g;
     g.A = e.X_f; g.Ys = e.Ys_f32;
     g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
-    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
+    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.raw = 0; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
   } else {
     Gemm1Args<double> g;
     g.A = e.X; g.Ys = e.Ys;
     g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
-    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.kf = kf; g.kr = kr; g.ky = ky;
+    g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.raw = 0; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
   }
+}
+
+void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *G, double *out,
+                      hipStream_t st) {
+  Gemm1Args<double> g;
+  g.A = e.X; g.Ys = M;
+  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = G; g.W = out;
+  g.Np = e.Np; g.npd = e.npd; g.do_q = 1; g.raw = 1; g.kf = g.kr = g.ky = 1.0;
+  hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
 }
 
 void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st) {
@@ -427,6 +553,19 @@ void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
   const size_t count = (size_t)sb * e.npd;
   hipLaunchKernelGGL(k_fav_reduce, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, e.Vp,
                      e.V, n_chunks, count);
+}
+
+void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, hipStream_t st) {
+  AdjArgs g;
+  g.X = e.X; g.Us = e.Us; g.Ys = e.Ys; g.W = e.W; g.Wu = e.Wu; g.G = e.G; g.Gu = e.Gu;
+  g.rho = e.rho; g.fups = e.fups; g.fpsi = e.fpsi; g.grad = e.gradp;
+  g.Np = e.Np; g.npd = e.npd; g.n = e.n; g.sb = sb; g.kf = kf; g.kr = kr; g.ky = ky;
+  hipLaunchKernelGGL(k_adj_grad, dim3(e.Np / BM, e.npd / BN), dim3(256), 0, st, g);
+}
+
+void launch_grad_out(const Ensemble &e, double *grad, int accumulate, hipStream_t st) {
+  hipLaunchKernelGGL(k_grad_out, dim3((e.n + 255) / 256, e.N), dim3(256), 0, st, e.gradp, grad,
+                     e.N, e.n, e.npd, accumulate);
 }
 
 }  // namespace tdscha

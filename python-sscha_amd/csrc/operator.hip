@@ -125,7 +125,59 @@ __global__ void k_post(const double *__restrict__ psi, double *__restrict__ out,
   out[ia] = rev * oa + shift * r;
 }
 
+// Adjoint seeds of the anharmonic part of k_post for a cotangent u:
+//   u . post(f_acc, d2v_acc) = uR . f_acc + <U, d2v_acc>  (U symmetric, zero padded),
+// with the sign, reverse and 1 / (N_eff N_sym) factors of k_post folded in.
+__global__ void k_post_adj(const double *__restrict__ u, double *__restrict__ U,
+                           double *__restrict__ uR, double inv_norm, const double *__restrict__ w,
+                           const double *__restrict__ nocc, int n, int npd, int flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (b >= npd) return;
+  const bool wig = flags & kOpWigner, tr = flags & kOpTranspose, anh = !(flags & kOpIgnoreV3);
+  const double rev = (flags & kOpReverse) ? -1.0 : 1.0;
+  if (a == 0) uR[b] = (anh && b < n) ? rev * (wig ? 1.0 : -1.0) * inv_norm * u[b] : 0.0;
+  double v = 0.0;
+  if (anh && a < n && b < n) {
+    const int i = a < b ? a : b, j = a < b ? b : a;
+    const long nw2 = (long)n * (n + 1) / 2;
+    const long p = pair_index(i, j, n);
+    const double uy = u[n + p], ua = u[n + nw2 + p];
+    const double wa = w[j], wb = w[i], na = nocc[j], nb = nocc[i];
+    double cy, ca;
+    if (wig) {
+      const double chi_m = ((wa - wb) * (na - nb)) / (2 * wa * wb);
+      const double chi_p = ((wa + wb) * (1 + na + nb)) / (2 * wa * wb);
+      cy = sqrt(-0.5 * chi_m);
+      ca = -sqrt(0.5 * chi_p);
+    } else {
+      const double ywa = 2 * wa / (2 * na + 1), ywb = 2 * wb / (2 * nb + 1);
+      if (!tr) {
+        const double rwa = 2 * wa * na * (na + 1) / (2 * na + 1);
+        const double rwb = 2 * wb * nb * (nb + 1) / (2 * nb + 1);
+        cy = -(ywb + ywa);
+        ca = -(rwb + rwa);
+      } else {
+        const double m = (i == j) ? 1.0 : 2.0;
+        cy = -m * (0.5 * (1 / ywb) * (1 / ywa));
+        ca = 0.0;
+      }
+    }
+    v = rev * inv_norm * (uy * cy + ua * ca);
+    if (i != j) v *= 0.5;   // the pair's entry is shared by (i, j) and (j, i)
+  }
+  U[(size_t)a * npd + b] = v;
+}
+
 }  // namespace
+
+void launch_post_adj(const Ensemble &e, const double *u, double *U, double *uR, int flags,
+                     hipStream_t st) {
+  double inv = 1.0 / (e.N_eff * (double)e.N_sym);
+  dim3 grid((e.npd + 255) / 256, e.npd);
+  hipLaunchKernelGGL(k_post_adj, grid, dim3(256), 0, st, u, U, uR, inv, e.w, e.nocc, e.n, e.npd,
+                     flags);
+}
 
 void launch_unpack(const Ensemble &e, int chain, const double *psi, int flags, hipStream_t st) {
   dim3 grid((e.npd + 255) / 256, e.npd);

@@ -182,6 +182,7 @@ __device__ inline void store_ys(double *Ys, void *lp, int lp_prec, size_t lp_pla
 }
 
 // grid (tile pairs g <= h, slot splits, chains); the workgroup rotates the slots
+// (optionally scaled: out[a][b] *= scale[a] scale[b])
 // t = blockIdx.y, blockIdx.y + gridDim.y, ... of its chain.
 template <int G>
 __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysym,
@@ -190,7 +191,8 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
                                                   const double *__restrict__ S,
                                                   const int2 *__restrict__ groups,
                                                   const int2 *__restrict__ pairs,
-                                                  const ModeMeta *__restrict__ meta, int npd,
+                                                  const ModeMeta *__restrict__ meta,
+                                                  const double *__restrict__ scale, int npd,
                                                   int sym_stride, Pass ps) {
   constexpr int LD = GrpTile<G>::LD, TPW = GrpTile<G>::TPW, NT = G / 16;
   __shared__ double M[G * LD], T[G * LD], Sg[G * LD], Sh[G * LD];
@@ -227,11 +229,10 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
       for (int r = 0; r < 4; ++r) {
         const int i = ti * 16 + lg + 4 * r, j = tj * 16 + lr;
         if (i < gg.y && j < gh.y) {
-          store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gg.x + i) * npd + gh.x + j,
-                   acc[q][r]);
+          const double v = scale ? acc[q][r] * scale[gg.x + i] * scale[gh.x + j] : acc[q][r];
+          store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gg.x + i) * npd + gh.x + j, v);
           if (pr.x != pr.y)
-            store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gh.x + j) * npd + gg.x + i,
-                     acc[q][r]);
+            store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gh.x + j) * npd + gg.x + i, v);
         }
       }
     }
@@ -326,8 +327,6 @@ __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G
   G[(size_t)sl * 2 * npd + npd + a] = g2;
 }
 
-// V[s][j] = sum_i YT[j][i] e1[s][i] + XtT[j][i] e2[s][i]
-// Block: 4 waves x 4 rows each = 16 rows j; lanes stride over configs i.
 // f_acc[chain][a] += sum over the chain's slots of sum_k S_s[a'][k] V[slot][off_A + k]
 __global__ void k_fav_rot_acc(const double *__restrict__ V, double *__restrict__ f_acc,
                               const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
@@ -415,7 +414,10 @@ void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pas
                      e.sym_stride, ps, (int)transS, (int)accumulate);
 }
 
-void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st) {
+namespace {
+
+void rot1_group(const Ensemble &e, const Pass &ps, const double *in, double *out64, void *lp,
+                int prec, const double *scale, hipStream_t st) {
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
   // enough workgroups to fill the chip: split each chain's slots over gridDim.y
   const long tiles = (long)e.n_pairs * ps.nc;
@@ -423,13 +425,41 @@ void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t 
   if (split > ps.ns) split = ps.ns;
   if (split < 1) split = 1;
   dim3 grid(e.n_pairs, split, ps.nc);
-  double *ys = prec == kFP64 ? e.Ys : (double *)nullptr;
   if (e.grp_width == 32)
-    hipLaunchKernelGGL(k_rot1_grp<32>, grid, dim3(256), 0, st, e.Ysym, ys, e.ys_lp(prec), prec,
-                       plane, e.S, e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+    hipLaunchKernelGGL(k_rot1_grp<32>, grid, dim3(256), 0, st, in, out64, lp, prec, plane, e.S,
+                       e.dgroups, e.dpairs, e.meta, scale, e.npd, e.sym_stride, ps);
   else
-    hipLaunchKernelGGL(k_rot1_grp<64>, grid, dim3(256), 0, st, e.Ysym, ys, e.ys_lp(prec), prec,
-                       plane, e.S, e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+    hipLaunchKernelGGL(k_rot1_grp<64>, grid, dim3(256), 0, st, in, out64, lp, prec, plane, e.S,
+                       e.dgroups, e.dpairs, e.meta, scale, e.npd, e.sym_stride, ps);
+}
+
+__global__ void k_scale_sym(double *M, const double *__restrict__ scale, int npd, size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const size_t r = (i / npd) % npd, c = i % npd;
+  M[i] *= scale[r] * scale[c];
+}
+
+}  // namespace
+
+void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st) {
+  rot1_group(e, ps, e.Ysym, prec == kFP64 ? e.Ys : (double *)nullptr, e.ys_lp(prec), prec,
+             nullptr, st);
+}
+
+void launch_rot_sym(const Ensemble &e, const Pass &ps, const double *in, double *out,
+                    const double *scale, hipStream_t st) {
+  if (e.fused_rot) {
+    rot1_group(e, ps, in, out, nullptr, kFP64, scale, st);
+  } else {
+    launch_rot_right(e, in, true, false, e.Tw, ps, false, st);
+    launch_rot_left(e, e.Tw, out, ps, true, false, st);
+    if (scale) {
+      const size_t count = (size_t)ps.slots() * e.npd * e.npd;
+      hipLaunchKernelGGL(k_scale_sym, grid1(count, 256), dim3(256), 0, st, out, scale, e.npd,
+                         count);
+    }
+  }
 }
 
 void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st) {
@@ -443,9 +473,14 @@ void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st) {
 }
 
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {
+  launch_rot_vec_from(e, ps, e.R1p, e.G, e.fpsi, st);
+}
+
+void launch_rot_vec_from(const Ensemble &e, const Pass &ps, const double *in, double *out,
+                         const double *div, hipStream_t st) {
   dim3 grid((e.npd + 255) / 256, ps.slots());
-  hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, e.R1p, e.G, e.S, e.meta, e.fups, e.fpsi,
-                     e.n, e.npd, e.sym_stride, ps);
+  hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, in, out, e.S, e.meta, e.fups, div, e.n,
+                     e.npd, e.sym_stride, ps);
 }
 
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st) {

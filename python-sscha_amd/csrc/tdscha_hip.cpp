@@ -583,6 +583,91 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *h, int precision, int n_chains,
   return 0;
 }
 
+// ---- force adjoint -------------------------------------------------------------------
+
+namespace {
+
+// Buffers of the force adjoint, allocated on its first call (and grown with the chains).
+int ensure_adjoint(tdscha_ensemble *h, int nc) {
+  Ensemble &e = h->e;
+  const size_t nn = (size_t)e.npd * e.npd, sb = (size_t)e.s_batch;
+  int rc = 0;
+  if (e.s_batch > e.adj_slots) {   // per-slot buffers follow the slot workspace
+    rc |= dev_alloc(h, &e.Us, sb * nn);
+    rc |= dev_alloc(h, &e.Gu, sb * 2 * e.npd);
+    rc |= dev_alloc(h, &e.Wu, sb * 4 * (size_t)e.Np);
+    if (rc) return rc;
+    e.adj_slots = e.s_batch;
+  }
+  if (!e.gradp) {
+    rc |= dev_alloc(h, &e.ones, (size_t)e.npd);
+    rc |= dev_alloc(h, &e.gradp, (size_t)e.Np * e.npd);
+    if (rc) return rc;
+    std::vector<double> one(e.npd, 1.0);
+    if (hipMemcpy(e.ones, one.data(), e.npd * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("upload failed");
+      return 10;
+    }
+  }
+  if (nc > e.adj_chains) {
+    rc |= dev_alloc(h, &e.Uc, (size_t)nc * nn);
+    rc |= dev_alloc(h, &e.uRc, (size_t)nc * e.npd);
+    if (rc) return rc;
+    e.adj_chains = nc;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int tdscha_ensemble_force_grad(tdscha_ensemble *h, int n_pairs, const int *flags,
+                               const double *const *u, const double *const *v, double *grad,
+                               int accumulate, void *stream) {
+  if (!h || !flags || !u || !v || !grad) { set_error("NULL argument"); return 1; }
+  if (n_pairs < 1 || n_pairs > TDSCHA_MAX_CHAINS) {
+    set_error("n_pairs must be in [1, " + std::to_string(TDSCHA_MAX_CHAINS) + "]");
+    return 1;
+  }
+  for (int c = 0; c < n_pairs; ++c) {
+    if (!u[c] || !v[c]) { set_error("NULL u / v"); return 1; }
+    if ((flags[c] & kOpIgnoreV4) != (flags[0] & kOpIgnoreV4)) {
+      set_error("IGNORE_V4 must be the same for every pair");
+      return 1;
+    }
+  }
+  Ensemble &e = h->e;
+  TD_CHECK(hipSetDevice(e.device));
+  int rc = ensure_chains(h, n_pairs);
+  if (!rc) rc = ensure_adjoint(h, n_pairs);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  launch_zero(e.gradp, (size_t)e.Np * e.npd, st);
+  for (int c = 0; c < n_pairs; ++c) {
+    launch_unpack(e, c, v[c], flags[c], st);
+    launch_post_adj(e, u[c], e.Uc + (size_t)c * e.npd * e.npd, e.uRc + (size_t)c * e.npd,
+                    flags[c], st);
+  }
+  const double ky = (flags[0] & kOpIgnoreV4) ? 0.0 : 1.0;
+  const int what = kTermF | kTermD2vR | (ky != 0.0 ? kTermD2vY : 0);
+  const int ns_max = e.s_batch / n_pairs > 0 ? e.s_batch / n_pairs : 1;
+  for (int s0 = 0; s0 < e.N_sym; s0 += ns_max) {
+    const Pass ps{s0, (e.N_sym - s0) < ns_max ? (e.N_sym - s0) : ns_max, n_pairs};
+    const int sb = ps.slots();
+    // forward quantities of the images: Ys, (g1, g2), weights c1, c2, e1, e2
+    launch_rot_sym(e, ps, e.Ysym, e.Ys, nullptr, st);
+    launch_rot_vec(e, ps, st);
+    launch_gemm1(e, sb, kFP64, what, st);
+    // seeds on the images: f_ups S^T U S f_ups, (f_ups S^T uR, S^T uR), then mu, nu
+    launch_rot_sym(e, ps, e.Uc, e.Us, e.fups, st);
+    launch_rot_vec_from(e, ps, e.uRc, e.Gu, e.ones, st);
+    launch_gemm1_raw(e, sb, e.Us, e.Gu, e.Wu, st);
+    launch_adj_grad(e, sb, 1.0, 1.0, ky, st);
+  }
+  launch_grad_out(e, grad, accumulate, st);
+  TD_CHECK(hipGetLastError());
+  return 0;
+}
+
 // ---- host-pointer drop-ins --------------------------------------------------------------
 
 static int host_call(const double *X, const double *Y, const double *w, const double *rho,

@@ -109,6 +109,16 @@ struct Ensemble {
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
   double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
   double *Pw = nullptr;       // [s_batch x col tiles x 4 x Np]  BF16x3 GEMM1 row sums
+  // force adjoint (allocated by its first call): per-chain seeds, per-slot rotated seeds,
+  // their row sums and the gradient accumulator
+  int adj_chains = 0, adj_slots = 0;
+  double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
+  double *uRc = nullptr;      // [chains x npd]        seed of f_av
+  double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
+  double *Gu = nullptr;       // [s_batch x 2 x npd]   f_ups S^T uR, S^T uR
+  double *Wu = nullptr;       // [s_batch x 4 x Np]    mu, -, nu, - per image
+  double *ones = nullptr;     // [npd]
+  double *gradp = nullptr;    // [Np x npd]            dJ/dY, padded
 };
 
 void set_error(const std::string &msg);
@@ -120,6 +130,12 @@ void launch_rot_right(const Ensemble &e, const double *in, bool per_chain, bool 
 void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pass &ps, bool transS,
                      bool accumulate, hipStream_t st);
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st);
+// out[slot] = (f_ups * S^T in[chain], S^T in[chain] / div)
+void launch_rot_vec_from(const Ensemble &e, const Pass &ps, const double *in, double *out,
+                         const double *div, hipStream_t st);
+// out[slot] = S^T in[chain] S (FP64), optionally scaled out[a][b] *= scale[a] scale[b]
+void launch_rot_sym(const Ensemble &e, const Pass &ps, const double *in, double *out,
+                    const double *scale, hipStream_t st);
 // Ys (fp64 when prec is FP64, else the reduced-precision planes) = S^T Ysym S, fused
 void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st);
 // d2v_acc[chain] += sum over the chain's slots of S (B_s + B_s^T) S^T, fused
@@ -136,8 +152,18 @@ void launch_convert(const double *src, void *dst, size_t count, int prec, hipStr
 void launch_unpack(const Ensemble &e, int chain, const double *psi, int flags, hipStream_t st);
 void launch_post(const Ensemble &e, int chain, const double *psi, double *out, int flags,
                  double shift, hipStream_t st);
+// seeds (U, uR) of u . (anharmonic part of post) for one chain
+void launch_post_adj(const Ensemble &e, const double *u, double *U, double *uR, int flags,
+                     hipStream_t st);
 
 // ---- GEMMs (gemm.hip) ----
+// GEMM1 with the rotated matrices M and vectors G of the slots, writing the four row sums
+// themselves (alpha, beta, gamma, delta) to out [sb x 4 x Np]
+void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *G, double *out,
+                      hipStream_t st);
+// ---- force adjoint (gemm.hip) ----
+void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, hipStream_t st);
+void launch_grad_out(const Ensemble &e, double *grad, int accumulate, hipStream_t st);
 // BF16x3 forms (gemm_bf16.hip)
 int bf16_col_tiles(const Ensemble &e);
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st);

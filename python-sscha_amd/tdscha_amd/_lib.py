@@ -48,6 +48,9 @@ SIGNATURES = {
     "tdscha_apply_full_L_batched": (ctypes.c_int, [
         _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "tdscha_ensemble_reserve": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tdscha_ensemble_force_grad": (ctypes.c_int, [
+        _vp, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, ctypes.c_int,
+        _vp]),
     "tdscha_lanczos_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, _dp]),

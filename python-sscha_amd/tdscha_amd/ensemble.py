@@ -172,3 +172,28 @@ class DeviceEnsemble:
                                                  _stream_handle(t, self.device))
         L.check(rc, "tdscha_apply_full_L_batched")
         return outs
+
+    def force_grad(self, us, vs, flags=None, out=None, accumulate=False):
+        """d/dY of sum_p u_p . A_p(v_p) (tdscha_ensemble_force_grad): A_p is the anharmonic
+        part of the operator with flags[p]. Returns an (N, n) FP64 device tensor."""
+        t = self.torch
+        n = self.n
+        C = len(us)
+        if len(vs) != C:
+            raise ValueError("us and vs must have the same length")
+        us = [self._dev(u, (n + n * (n + 1),)) for u in us]
+        vs = [self._dev(v, (n + n * (n + 1),)) for v in vs]
+        flags = [0] * C if flags is None else list(flags)
+        if out is None:
+            out = t.zeros((self.N, n), dtype=t.float64, device=self.device)
+            accumulate = False
+        elif tuple(out.shape) != (self.N, n) or out.dtype != t.float64 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 (N, n) tensor")
+        fl = (ctypes.c_int * C)(*[int(f) for f in flags])
+        pu = (ctypes.c_void_p * C)(*[u.data_ptr() for u in us])
+        pv = (ctypes.c_void_p * C)(*[v.data_ptr() for v in vs])
+        rc = L.lib().tdscha_ensemble_force_grad(self._h, C, fl, pu, pv, out.data_ptr(),
+                                                int(bool(accumulate)),
+                                                _stream_handle(t, self.device))
+        L.check(rc, "tdscha_ensemble_force_grad")
+        return out
