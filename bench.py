@@ -83,6 +83,9 @@ def main():
                     help="vectors per batched application (alternating L and L^T)")
     ap.add_argument("--s-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the timed steps, copy a buffer of this many MiB on the device "
+                         "(a kernel of known traffic for hardware-counter calibration)")
     ap.add_argument("--fp64-sample-syms", type=int, default=-1,
                     help="symmetries of the FP64 comparison sample (-1: N_sym/32, 0: skip)")
     args = ap.parse_args()
@@ -142,6 +145,14 @@ def main():
     if not all(bool(torch.isfinite(o).all()) for o in outs):
         raise RuntimeError("non-finite output")
 
+    if args.copy_calibration_mb:
+        src = torch.ones(args.copy_calibration_mb * 131072, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read, %d MiB written" % (args.copy_calibration_mb,
+                                                                args.copy_calibration_mb))
+        del src, dst
     groups = {name: L.profile_query(gid) for gid, name in L.KGROUP_NAMES.items()}
     log("kernel groups (ms total, launches): %s" % groups)
     M = N * ens.n_syms

@@ -36,7 +36,7 @@ extern "C" {
 
 /* Arithmetic of the two ensemble contractions. */
 #define TDSCHA_PREC_FP64 0   /* FP64 MFMA, the plain path */
-#define TDSCHA_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation in FP64-flushed chunks */
+#define TDSCHA_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
 #define TDSCHA_PREC_BF16X3 2 /* hi*hi + hi*lo + lo*hi BF16 MFMA, FP32 accumulation */
 
 /* Terms of the perturbed averages (bit mask). */
