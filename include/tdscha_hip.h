@@ -160,6 +160,14 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
                                 const int *flags, const double *shifts,
                                 const double *const *psi, double *const *out, void *stream);
 
+/* Pass-size autotuner (no reference counterpart): times two passes of the batched
+ * L-application for n_chains chains at slot counts s_batch, s_batch/2, ... (at most six)
+ * and keeps the fastest per image for this ensemble. Returns the chosen slot count and its
+ * time per (chain x configuration x symmetry) image in microseconds (either may be NULL).
+ * Call before capturing Lanczos graphs on the ensemble. */
+int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, int *slots,
+                             double *us_per_image);
+
 /* Allocate the per-chain buffers for n_chains chains now (what the first batched call with
  * that many chains would do), so that later calls allocate nothing. */
 int tdscha_ensemble_reserve(tdscha_ensemble *ens, int n_chains);
@@ -199,6 +207,17 @@ int tdscha_lanczos_read(tdscha_lanczos *lz, int chain, double *a, double *b, dou
                         double *s_norm, int *steps_done);
 /* Device pointers to the stored basis of a chain ((max_steps + 1) x (n + n (n + 1))). */
 int tdscha_lanczos_basis(tdscha_lanczos *lz, int chain, const double **Q, const double **P);
+
+/* Several Lanczos runs on different ensembles (systems of uneven size) advanced together:
+ * one graph per step holds the step of every member, each on a branch of its own, so the
+ * kernels of a small system run beside those of a large one. Members stay usable on their
+ * own (read, basis); a group must not be run while a member is run alone. No reference
+ * counterpart (the reference runs one Lanczos at a time). */
+typedef struct tdscha_lanczos_group tdscha_lanczos_group;
+int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *const *members,
+                                int n_members);
+int tdscha_lanczos_group_run(tdscha_lanczos_group *g, int n_steps, void *stream);
+int tdscha_lanczos_group_destroy(tdscha_lanczos_group *g);
 
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;

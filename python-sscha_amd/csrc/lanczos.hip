@@ -460,6 +460,102 @@ int tdscha_lanczos_run(tdscha_lanczos *z, int n_steps, int use_graph, void *stre
   return 0;
 }
 
+// ---- several Lanczos runs (one per ensemble) in one graph per step --------------------
+// The step of every member is captured on a branch of its own (fork / join through events
+// on default-priority streams), so the kernels of a small system can run beside those of a
+// large one; one graph launch advances every member by one step.
+struct tdscha_lanczos_group {
+  std::vector<tdscha_lanczos *> members;
+  std::vector<hipStream_t> branches;
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int tdscha_lanczos_group_destroy(tdscha_lanczos_group *g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (hipStream_t s : g->branches) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : g->joins) (void)hipEventDestroy(e);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->cap) (void)hipStreamDestroy(g->cap);
+  delete g;
+  return 0;
+}
+
+int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *const *members,
+                                int n_members) {
+  if (!out || !members || n_members < 1) { set_error("bad argument"); return 1; }
+  for (int i = 0; i < n_members; ++i) {
+    if (!members[i]) { set_error("NULL member"); return 1; }
+    for (int j = 0; j < i; ++j)
+      if (members[j]->ens == members[i]->ens) {
+        set_error("members of a group must use different ensembles (they share its workspace)");
+        return 1;
+      }
+    if (members[i]->device != members[0]->device) {
+      set_error("members of a group must be on one device");
+      return 1;
+    }
+  }
+  tdscha_lanczos_group *g = new (std::nothrow) tdscha_lanczos_group();
+  if (!g) { set_error("out of host memory"); return 11; }
+  g->members.assign(members, members + n_members);
+  auto fail = [&](const char *what, hipError_t e) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    tdscha_lanczos_group_destroy(g);
+    return 10;
+  };
+  hipError_t e = hipSetDevice(members[0]->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+  for (int i = 0; i < n_members && e == hipSuccess; ++i) {
+    hipStream_t s = nullptr;
+    hipEvent_t j = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e == hipSuccess) g->branches.push_back(s);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&j, hipEventDisableTiming);
+    if (e == hipSuccess) g->joins.push_back(j);
+  }
+  if (e != hipSuccess) return fail("stream / event creation failed", e);
+  tdscha_profile_enable(0);  // no timing events inside a captured graph
+  e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail("stream capture failed", e);
+  int rc = 0;
+  e = hipEventRecord(g->fork, g->cap);
+  for (int i = 0; i < n_members && e == hipSuccess && !rc; ++i) {
+    e = hipStreamWaitEvent(g->branches[i], g->fork, 0);
+    if (e == hipSuccess) rc = enqueue_step(g->members[i], g->branches[i]);
+    if (e == hipSuccess && !rc) e = hipEventRecord(g->joins[i], g->branches[i]);
+    if (e == hipSuccess && !rc) e = hipStreamWaitEvent(g->cap, g->joins[i], 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(g->cap, &graph);
+  g->graph = graph;
+  if (rc) { tdscha_lanczos_group_destroy(g); return rc; }
+  if (e != hipSuccess) return fail("capturing the group step failed", e);
+  if (e2 != hipSuccess) return fail("end capture failed", e2);
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) return fail("graph instantiation failed", e);
+  *out = g;
+  return 0;
+}
+
+int tdscha_lanczos_group_run(tdscha_lanczos_group *g, int n_steps, void *stream) {
+  if (!g || n_steps < 0) { set_error("bad argument"); return 1; }
+  for (int i = 0; i < n_steps; ++i) {
+    hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (e != hipSuccess) {
+      set_error(std::string("graph launch failed: ") + hipGetErrorString(e));
+      return 10;
+    }
+  }
+  return 0;
+}
+
 int tdscha_lanczos_read(tdscha_lanczos *z, int chain, double *a, double *b, double *c,
                         double *s_norm, int *steps_done) {
   if (!z || chain < 0 || chain >= z->nc || !steps_done) { set_error("bad argument"); return 1; }

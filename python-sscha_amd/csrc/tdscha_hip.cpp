@@ -169,7 +169,9 @@ struct Span {
 // Per-pass orchestration of one ensemble application for nc chains. Y1/R1 of chain c must
 // already be staged in e.Ysym / e.R1p + c (prep or unpack kernel). Leaves unnormalised sums
 // in f_acc / d2v_acc of every chain.
-int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st) {
+int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st, int s_begin = 0,
+               int s_end = -1) {
+  if (s_end < 0) s_end = e.N_sym;
   const bool need_q = what & (kTermF | kTermD2vY);
   const bool need_r = what & kTermD2vR;
   const bool need_d = what & (kTermD2vR | kTermD2vY);
@@ -177,9 +179,10 @@ int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st) {
   launch_zero(e.f_acc, (size_t)nc * e.npd, st);
   launch_zero(e.d2v_acc, (size_t)nc * e.npd * e.npd, st);
   if (!need_r) launch_zero(e.G, (size_t)e.s_batch * 2 * e.npd, st);
-  const int ns_max = e.s_batch / nc > 0 ? e.s_batch / nc : 1;
-  for (int s0 = 0; s0 < e.N_sym; s0 += ns_max) {
-    const Pass ps{s0, (e.N_sym - s0) < ns_max ? (e.N_sym - s0) : ns_max, nc};
+  const int slots = (e.pass_slots > 0 && e.pass_slots < e.s_batch) ? e.pass_slots : e.s_batch;
+  const int ns_max = slots / nc > 0 ? slots / nc : 1;
+  for (int s0 = s_begin; s0 < s_end; s0 += ns_max) {
+    const Pass ps{s0, (s_end - s0) < ns_max ? (s_end - s0) : ns_max, nc};
     const int sb = ps.slots();
     {
       Span sp(TDSCHA_KGROUP_ROTATE, st);
@@ -580,6 +583,51 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *h, int precision, int n_chains,
   }
   for (int c = 0; c < n_chains; ++c) launch_post(e, c, psi[c], out[c], flags[c], shifts[c], st);
   TD_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---- pass-size autotuner ---------------------------------------------------------------
+
+int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, int *slots_out,
+                             double *us_per_image) {
+  if (!h || n_chains < 1 || n_chains > TDSCHA_MAX_CHAINS) { set_error("bad argument"); return 1; }
+  Ensemble &e = h->e;
+  int rc = check_prec(e, precision);
+  if (rc) return rc;
+  TD_CHECK(hipSetDevice(e.device));
+  rc = ensure_chains(h, n_chains);
+  if (rc) return rc;
+  const bool prof = g_prof.on;
+  g_prof.on = false;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  TD_CHECK(hipEventCreate(&t0));
+  TD_CHECK(hipEventCreate(&t1));
+  const int what = kTermF | kTermD2vR | kTermD2vY;
+  int best = e.s_batch;
+  double best_t = 1e300;
+  for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
+    e.pass_slots = c;
+    const int ns = c / n_chains;
+    const int s_end = (2 * ns < e.N_sym) ? 2 * ns : e.N_sym;   // two passes (or all images)
+    rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);   // warm-up
+    if (!rc && hipEventRecord(t0, nullptr) != hipSuccess) rc = 10;
+    if (!rc) rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);
+    if (!rc && hipEventRecord(t1, nullptr) != hipSuccess) rc = 10;
+    float ms = 0.f;
+    if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+      rc = 10;
+    if (rc) break;
+    const double per = 1e3 * ms / ((double)s_end * n_chains * e.N);
+    if (per < best_t) { best_t = per; best = c; }
+    if (s_end == e.N_sym && c < e.s_batch) break;   // smaller passes cannot fill more
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  g_prof.on = prof;
+  if (rc) { e.pass_slots = 0; if (rc == 10) set_error("autotune timing failed"); return rc; }
+  e.pass_slots = best;
+  if (slots_out) *slots_out = best;
+  if (us_per_image) *us_per_image = best_t;
   return 0;
 }
 

@@ -112,6 +112,7 @@ struct Ensemble {
   // force adjoint (allocated by its first call): per-chain seeds, per-slot rotated seeds,
   // their row sums and the gradient accumulator
   int adj_chains = 0, adj_slots = 0;
+  int pass_slots = 0;         // slots used per pass (autotuned; 0 = s_batch)
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups

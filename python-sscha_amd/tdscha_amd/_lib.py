@@ -48,6 +48,7 @@ SIGNATURES = {
     "tdscha_apply_full_L_batched": (ctypes.c_int, [
         _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "tdscha_ensemble_reserve": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tdscha_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _dp]),
     "tdscha_ensemble_force_grad": (ctypes.c_int, [
         _vp, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, ctypes.c_int,
         _vp]),
@@ -60,6 +61,10 @@ SIGNATURES = {
     "tdscha_lanczos_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
     "tdscha_lanczos_basis": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp),
                                             ctypes.POINTER(_vp)]),
+    "tdscha_lanczos_group_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                   ctypes.c_int]),
+    "tdscha_lanczos_group_run": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "tdscha_lanczos_group_destroy": (ctypes.c_int, [_vp]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

@@ -173,6 +173,16 @@ class DeviceEnsemble:
         L.check(rc, "tdscha_apply_full_L_batched")
         return outs
 
+    def autotune(self, precision="bf16x3", n_chains=2):
+        """Pick the slot count per pass by timing (tdscha_ensemble_autotune); returns
+        (slots, microseconds per chain-configuration-symmetry image)."""
+        slots = ctypes.c_int()
+        per = ctypes.c_double()
+        L.check(L.lib().tdscha_ensemble_autotune(self._h, L.PRECISIONS[precision], int(n_chains),
+                                                 ctypes.byref(slots), ctypes.byref(per)),
+                "tdscha_ensemble_autotune")
+        return slots.value, per.value
+
     def force_grad(self, us, vs, flags=None, out=None, accumulate=False):
         """d/dY of sum_p u_p . A_p(v_p) (tdscha_ensemble_force_grad): A_p is the anharmonic
         part of the operator with flags[p]. Returns an (N, n) FP64 device tensor."""

@@ -116,6 +116,37 @@ class DeviceLanczos:
         return Q, P
 
 
+class LanczosGroup:
+    """DeviceLanczos runs on different ensembles advanced together, one graph launch per
+    step (tdscha_lanczos_group_*). Start every member first."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.torch = self.members[0].torch
+        arr = (ctypes.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        h = ctypes.c_void_p()
+        L.check(L.lib().tdscha_lanczos_group_create(ctypes.byref(h), arr, len(self.members)),
+                "tdscha_lanczos_group_create")
+        self._h = h
+
+    def run(self, n_steps):
+        m = self.members[0]
+        L.check(L.lib().tdscha_lanczos_group_run(self._h, int(n_steps),
+                                                 _stream_handle(self.torch, m.ens.device)),
+                "tdscha_lanczos_group_run")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L.lib().tdscha_lanczos_group_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class _DeviceView:
     """A torch tensor over library-owned device memory (via __cuda_array_interface__)."""
 

@@ -228,6 +228,20 @@ class SyntheticEnsemble:
         v[:, direction] = np.where(self.crystal.species == 0, zeff, -zeff)
         return np.einsum("a,a,ab->b", 1 / np.sqrt(self.m), v.ravel(), self.pols)
 
+    def raman_like_perturbation(self, component, seed=7):
+        """R-sector perturbation of one polarizability component (0..5: xx, yy, zz, xy, xz,
+        yz) from seeded synthetic per-atom Raman tensors d(alpha_ij)/dR_a, in the form of
+        prepare_raman (Modules/DynamicalLanczos.py:1693-1781). Rock salt has no first-order
+        Raman activity, so these tensors are not symmetric: they only give twelve-chain
+        batches realistic, distinct starting vectors."""
+        nat = self.crystal.nat
+        pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+        i, j = pairs[component]
+        t = np.random.default_rng(seed).standard_normal((nat, 3, 3, 3))
+        t = 0.5 * (t + t.transpose(0, 2, 1, 3))      # symmetric in the field indices
+        v = t[:, i, j, :]
+        return np.einsum("a,a,ab->b", 1 / np.sqrt(self.m), v.ravel(), self.pols)
+
 
 def modes(cr: Crystal, phi, tol=1e-7):
     """Frequencies (Ry), mass-scaled polarisation vectors and degenerate blocks, translations

@@ -115,3 +115,47 @@ def test_lanczos_class_device_loop(mode10):
     lz.run_FT(10, verbose=False, device_loop=True)
     assert abs(lz.c_coeffs[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
     assert len(lz.basis_Q) == len(lz.b_coeffs) + 1
+
+
+def test_group_of_uneven_systems_matches_members_alone():
+    """Three ensembles of different sizes advanced by one graph per step give bit-identical
+    coefficients to each run on its own."""
+    from tdscha_amd.lanczos_device import DeviceLanczos, LanczosGroup
+    sizes = [(30, 40, 3, 61), (52, 70, 5, 62), (75, 33, 2, 63)]
+    ens = [fixtures.random_ensemble(seed, n, N, ns) for n, N, ns, seed in sizes]
+    devs = [_dev_ens(e, precisions=("fp64", "bf16x3")) for e in ens]
+    rng = np.random.default_rng(64)
+    starts = [[rng.standard_normal(orc.psi_len(n)) for _ in range(2)] for n, _, _, _ in sizes]
+    members = []
+    for d, ps in zip(devs, starts):
+        m = DeviceLanczos(d, 2, "bf16x3", max_steps=6)
+        m.start(ps)
+        members.append(m)
+    group = LanczosGroup(members)
+    group.run(6)
+    for d, ps, m in zip(devs, starts, members):
+        alone = DeviceLanczos(d, 2, "bf16x3", max_steps=6)
+        alone.start(ps)
+        alone.run(6)
+        for ch in range(2):
+            for x, y in zip(m.coefficients(ch, truncated=False),
+                            alone.coefficients(ch, truncated=False)):
+                assert np.array_equal(x, y)
+        alone.close()
+    group.close()
+
+
+def test_autotuned_pass_size_keeps_results():
+    """The autotuner only changes how the images are split into passes; the per-image
+    contributions are added in the same order, so L is bit-identical."""
+    import torch
+    ens = fixtures.random_ensemble(65, 60, 90, 24)
+    dev = _dev_ens(ens, precisions=("fp64", "bf16x3"), s_batch=24)
+    psi = torch.as_tensor(np.random.default_rng(66).standard_normal(orc.psi_len(60)),
+                          device=dev.device)
+    before = [dev.apply_full_L(psi, precision=p).clone() for p in ("fp64", "bf16x3")]
+    slots, per = dev.autotune("bf16x3", 2)
+    assert 2 <= slots <= 24 and per > 0
+    after = [dev.apply_full_L(psi, precision=p) for p in ("fp64", "bf16x3")]
+    for x, y in zip(before, after):
+        assert torch.equal(x, y)
