@@ -7,7 +7,9 @@ A step is one batched L application (tdscha_apply_full_L_batched) to the pair a 
 step needs, L q and L^T p, on seeded random unit vectors: for each chain the harmonic part,
 the pre-transform, the ensemble average over all 20000 x 10368 images and the
 post-transform, with both chains sharing every pass over the symmetries. Every input is
-resident in HBM; value counts L applications (2 per step) per second.
+resident in HBM; value counts L applications (2 per step) per second. Before the warm-up the
+autotuner picks the contraction tile shape and the image slots per pass for this ensemble
+(timed on two passes per candidate, outside the timed region).
 
 Prints one JSON line. `roofline` is measured live with HIP events around every launch of
 each kernel group over the timed steps; `cpu_baseline` times the reference's own C trio
@@ -83,6 +85,8 @@ def main():
                     help="vectors per batched application (alternating L and L^T)")
     ap.add_argument("--s-batch", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile shape and pass size instead of timing them")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB on the device "
                          "(a kernel of known traffic for hardware-counter calibration)")
@@ -127,6 +131,12 @@ def main():
     def step():
         dens.apply_full_L_batched(psis, flags, shifts, precision=args.precision, outs=outs)
 
+    if not args.no_autotune:
+        t0 = time.time()
+        slots, per = dens.autotune(args.precision, args.chains)
+        log("autotune: tiles %d, %d slots per pass (%.3g us per image) in %.1f s" % (
+            dens.tuning()[0], slots, per, time.time() - t0))
+
     for k in range(args.warmup):
         step()
         torch.cuda.synchronize()
@@ -153,6 +163,7 @@ def main():
         log("calibration copy: %d MiB read, %d MiB written" % (args.copy_calibration_mb,
                                                                 args.copy_calibration_mb))
         del src, dst
+    dens_tuning = dens.tuning()
     groups = {name: L.profile_query(gid) for gid, name in L.KGROUP_NAMES.items()}
     log("kernel groups (ms total, launches): %s" % groups)
     M = N * ens.n_syms
@@ -216,6 +227,7 @@ def main():
                                "(%s)" % (Lc, Lc, Lc, 2 * Lc ** 3, n, N, ens.n_syms, args.chains,
                                          args.precision),
                    "chains": args.chains,
+                   "bf16_tiles": dens_tuning[0], "slots_per_pass": dens_tuning[1],
                    "n_modes": n, "n_configs": N, "n_syms": ens.n_syms,
                    "flop_per_L_application": flop_l,
                    "achieved_TFLOPs_whole_L": round(flop_l * value / 1e12, 3)},

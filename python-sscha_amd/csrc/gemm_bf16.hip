@@ -4,12 +4,14 @@
 // three v_mfma_f32_16x16x32_bf16 per operand pair, FP32 accumulation, FP64 epilogues.
 //
 // Both contractions are "NT" products C[m][c] = sum_k A[m][k] B[c][k] with k-contiguous
-// operand rows, tiled R x 128 per workgroup (4 waves as 2 x 2, each R/2 x 64 = R/32 x 4
-// MFMA blocks; R = 256 for GEMM1, 192 for GEMM2, whose operand formation needs the
-// registers), k in slices of 32 staged through LDS as [row][k] with an 80-byte row stride
+// operand rows, tiled TM x TN per workgroup of 2 TN threads (waves in a 2 x TN/64 grid, each
+// TM/2 x 64 = TM/32 x 4 MFMA blocks). Two shapes, picked per ensemble by the autotuner:
+// 256 x 128 (GEMM1) / 192 x 128 (GEMM2, whose operand formation needs the registers a
+// 256-row tile would take) with 4 waves, or 256 x 256 for both with 8 waves. k goes in
+// slices of 32 staged through LDS as [row][k] with an 80-byte row stride
 // (conflict-free 16-byte fragment reads). The next slice is fetched into registers while
-// the current one is multiplied. Sub-blocks past the valid rows / columns of a ragged tile
-// issue no MFMAs.
+// the current one is multiplied. Waves whose sub-tile lies past the valid rows / columns of
+// a ragged tile issue no MFMAs.
 //
 // Workgroups are numbered so that each XCD (they are dealt out round-robin by workgroup id)
 // runs a contiguous range of tiles, slot-major: the concurrent workgroups of an XCD work on
@@ -33,15 +35,21 @@ typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
 
-constexpr int TM1 = 256;         // GEMM1 workgroup tile rows (configurations)
-constexpr int TM2 = 192;         // GEMM2 workgroup tile rows (modes)
-constexpr int TN = kBfTileN;     // workgroup tile columns (128)
 constexpr int TK = 32;           // k slice
 constexpr int LDK = TK + 8;      // LDS row stride in bf16
 constexpr int WN = 4;            // 16 x 16 column blocks per wave (64 columns)
 
-template <int R> struct Stage {
-  uint16_t ah[R * LDK], al[R * LDK], bh[TN * LDK], bl[TN * LDK];
+// A workgroup tile is TM x TN with 2 TN threads: waves in a 2 x (TN / 64) grid, each
+// (TM / 2) x 64. Variant 0: GEMM1 256 x 128, GEMM2 192 x 128 (256 threads); variant 1:
+// both 256 x 256 (512 threads), half the operand traffic per flop of variant 0.
+template <int TM, int TN> struct Tile {
+  static constexpr int NT = 2 * TN;           // threads
+  static constexpr int WC = TN / 64;          // waves across the columns
+  static constexpr int WM = TM / 32;          // 16-row blocks per wave
+};
+
+template <int TM, int TN> struct Stage {
+  uint16_t ah[TM * LDK], al[TM * LDK], bh[TN * LDK], bl[TN * LDK];
 };
 
 // Workgroup id -> logical tile id, so that XCD x runs logical ids [x * per, (x + 1) * per).
@@ -61,36 +69,37 @@ __device__ inline void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
   lo = __builtin_bit_cast(uint32_t, l);
 }
 
-// A slice of R rows x 32 k of one bf16 plane: R / 64 16-byte chunks per thread, rows
-// t/4 + 64 h (zero past `rows`).
-template <int R>
+// R rows x 32 k of one bf16 plane with NT threads: 16-byte chunks, rows t/4 + (NT/4) h
+// (zero past `rows`).
+template <int R, int NT>
 __device__ inline void load_rows(const uint16_t *__restrict__ P, size_t ld, int rows, int r0,
-                                 int k0, uint4 (&reg)[R / 64]) {
+                                 int k0, uint4 (&reg)[4 * R / NT]) {
   const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
 #pragma unroll
-  for (int h = 0; h < R / 64; ++h) {
-    const int r = r0 + row + 64 * h;
+  for (int h = 0; h < 4 * R / NT; ++h) {
+    const int r = r0 + row + (NT / 4) * h;
     reg[h] = r < rows ? *reinterpret_cast<const uint4 *>(P + (size_t)r * ld + k0 + kc)
                       : make_uint4(0, 0, 0, 0);
   }
 }
 
-template <int R>
-__device__ inline void store_rows(uint16_t *lds, const uint4 (&reg)[R / 64]) {
+template <int R, int NT>
+__device__ inline void store_rows(uint16_t *lds, const uint4 (&reg)[4 * R / NT]) {
   const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
 #pragma unroll
-  for (int h = 0; h < R / 64; ++h)
-    *reinterpret_cast<uint4 *>(lds + (row + 64 * h) * LDK + kc) = reg[h];
+  for (int h = 0; h < 4 * R / NT; ++h)
+    *reinterpret_cast<uint4 *>(lds + (row + (NT / 4) * h) * LDK + kc) = reg[h];
 }
 
 __device__ inline v8bf frag(const uint16_t *lds, int row, int kg) {
   return __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4 *>(lds + row * LDK + kg * 8));
 }
 
-// 3 x (mi x nj) MFMAs of one slice for this wave; mi / nj = blocks with valid rows / columns.
-template <int R, int WM = R / 32>
-__device__ inline void mma_slice(const Stage<R> &s, int wr, int wc, int mi, int nj,
-                                 v4f (&acc)[WM][WN]) {
+// 3 MFMAs per 16 x 16 block of this wave's sub-tile, for one k slice.
+template <int TM, int TN>
+__device__ inline void mma_full(const Stage<TM, TN> &s, int wr, int wc,
+                                v4f (&acc)[Tile<TM, TN>::WM][WN]) {
+  constexpr int WM = Tile<TM, TN>::WM;
   const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   v8bf bh[WN], bl[WN];
 #pragma unroll
@@ -100,24 +109,31 @@ __device__ inline void mma_slice(const Stage<R> &s, int wr, int wc, int mi, int 
   }
 #pragma unroll
   for (int ti = 0; ti < WM; ++ti) {
-    if (ti < mi) {
-      const v8bf ah = frag(s.ah, wr * (R / 2) + ti * 16 + lr, lg);
-      const v8bf al = frag(s.al, wr * (R / 2) + ti * 16 + lr, lg);
+    const v8bf ah = frag(s.ah, wr * (TM / 2) + ti * 16 + lr, lg);
+    const v8bf al = frag(s.al, wr * (TM / 2) + ti * 16 + lr, lg);
 #pragma unroll
-      for (int tj = 0; tj < WN; ++tj) {
-        if (tj < nj) {
-          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
-          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
-          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
-        }
-      }
+    for (int tj = 0; tj < WN; ++tj) {
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
     }
   }
 }
 
+// mi / nj (blocks of this wave's sub-tile with valid rows / columns) are wave-uniform, read
+// from the first lane, so the branch is scalar: a wave entirely past the edge of a ragged
+// tile issues no MFMA; a partly valid one multiplies the zero padding (operand rows past
+// the edge are loaded as zeros) rather than carrying a second, guarded copy of the body.
+template <int TM, int TN>
+__device__ inline void mma_slice(const Stage<TM, TN> &s, int wr, int wc, int mi, int nj,
+                                 v4f (&acc)[Tile<TM, TN>::WM][WN]) {
+  if (mi > 0 && nj > 0) mma_full<TM, TN>(s, wr, wc, acc);
+}
+
 __device__ inline int clamp_blocks(int valid, int start, int cap) {
   int b = (valid - start + 15) / 16;
-  return b < 0 ? 0 : (b > cap ? cap : b);
+  b = b < 0 ? 0 : (b > cap ? cap : b);
+  return __builtin_amdgcn_readfirstlane(b);
 }
 
 // ------------------------------------------------------------------------------ GEMM1
@@ -130,10 +146,11 @@ struct Gemm1Args {
   int Np, npd, sb, n_it, n_ct, do_q;
 };
 
-__global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
-  constexpr int TM = TM1, WM = TM1 / 32;
-  __shared__ Stage<TM1> sm;
-  __shared__ double red[2][TM][4];
+template <int TM, int TN>
+__global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm1_bf16x3(Gemm1Args g) {
+  constexpr int NT = Tile<TM, TN>::NT, WC = Tile<TM, TN>::WC, WM = Tile<TM, TN>::WM;
+  __shared__ Stage<TM, TN> sm;
+  __shared__ double red[WC][TM][4];
   const int total = g.sb * g.n_it * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
@@ -143,7 +160,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
   const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
   const int i0 = it * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1, lr = l & 15, lg = l >> 4;
+  const int wr = w / WC, wc = w % WC, lr = l & 15, lg = l >> 4;
   const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
   const int mi = WM;   // Np is a multiple of the row tile
   const size_t plane_off = (size_t)sl * g.npd * g.npd;
@@ -156,26 +173,26 @@ __global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
 
   if (g.do_q) {
     const int KT = g.npd / TK;
-    uint4 rah[TM / 64], ral[TM / 64], rbh[TN / 64], rbl[TN / 64];
-    load_rows<TM>(g.Xh, g.npd, g.Np, i0, 0, rah);
-    load_rows<TM>(g.Xl, g.npd, g.Np, i0, 0, ral);
-    load_rows<TN>(g.Yh + plane_off, g.npd, g.npd, c0, 0, rbh);
-    load_rows<TN>(g.Yl + plane_off, g.npd, g.npd, c0, 0, rbl);
+    uint4 rah[4 * TM / NT], ral[4 * TM / NT], rbh[4 * TN / NT], rbl[4 * TN / NT];
+    load_rows<TM, NT>(g.Xh, g.npd, g.Np, i0, 0, rah);
+    load_rows<TM, NT>(g.Xl, g.npd, g.Np, i0, 0, ral);
+    load_rows<TN, NT>(g.Yh + plane_off, g.npd, g.npd, c0, 0, rbh);
+    load_rows<TN, NT>(g.Yl + plane_off, g.npd, g.npd, c0, 0, rbl);
     for (int kt = 0; kt < KT; ++kt) {
       __syncthreads();
-      store_rows<TM>(sm.ah, rah);
-      store_rows<TM>(sm.al, ral);
-      store_rows<TN>(sm.bh, rbh);
-      store_rows<TN>(sm.bl, rbl);
+      store_rows<TM, NT>(sm.ah, rah);
+      store_rows<TM, NT>(sm.al, ral);
+      store_rows<TN, NT>(sm.bh, rbh);
+      store_rows<TN, NT>(sm.bl, rbl);
       __syncthreads();
       if (kt + 1 < KT) {
         const int k0 = (kt + 1) * TK;
-        load_rows<TM>(g.Xh, g.npd, g.Np, i0, k0, rah);
-        load_rows<TM>(g.Xl, g.npd, g.Np, i0, k0, ral);
-        load_rows<TN>(g.Yh + plane_off, g.npd, g.npd, c0, k0, rbh);
-        load_rows<TN>(g.Yl + plane_off, g.npd, g.npd, c0, k0, rbl);
+        load_rows<TM, NT>(g.Xh, g.npd, g.Np, i0, k0, rah);
+        load_rows<TM, NT>(g.Xl, g.npd, g.Np, i0, k0, ral);
+        load_rows<TN, NT>(g.Yh + plane_off, g.npd, g.npd, c0, k0, rbh);
+        load_rows<TN, NT>(g.Yl + plane_off, g.npd, g.npd, c0, k0, rbl);
       }
-      mma_slice<TM>(sm, wr, wc, mi, nj, acc);
+      mma_slice<TM, TN>(sm, wr, wc, mi, nj, acc);
     }
   }
 
@@ -193,7 +210,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
         const double gg1 = g1[c], gg2 = g2[c];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = wr * 128 + ti * 16 + 4 * lg + r;
+          const int m = wr * (TM / 2) + ti * 16 + 4 * lg + r;
           const size_t o = (size_t)(i0 + m) * g.npd + c;
           const double x = g.Xf[o], ft = g.Ftf[o];
           const double q = (double)acc[ti][tj][r];
@@ -211,14 +228,17 @@ __global__ __launch_bounds__(256, 2) void k_gemm1_bf16x3(Gemm1Args g) {
         double v = p[r][q];
 #pragma unroll
         for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-        if (lr == 0) red[wc][wr * 128 + ti * 16 + 4 * lg + r][q] = v;
+        if (lr == 0) red[wc][wr * (TM / 2) + ti * 16 + 4 * lg + r][q] = v;
       }
   }
   __syncthreads();
   double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np + i0;
   for (int idx = threadIdx.x; idx < TM * 4; idx += blockDim.x) {
     const int q = idx / TM, m = idx - q * TM;
-    Pt[(size_t)q * g.Np + m] = red[0][m][q] + red[1][m][q];
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < WC; ++c) v += red[c][m][q];
+    Pt[(size_t)q * g.Np + m] = v;
   }
 }
 
@@ -251,13 +271,15 @@ struct Gemm2Args {
   int Np, npd, sb, n_mt, n_ct;
 };
 
-// G^T slice (rows c0 + t/4 + 64 h, k0 + 8 (t % 4) ..): formed in FP32 and split.
-__device__ inline void load_g(const Gemm2Args &g, const double *c1, const double *c2, int c0,
-                              int k0, float4 (&ry)[4], float4 (&rx)[4]) {
+// G^T slice of TN rows (c0 + t/4 + (NT/4) h, h = 0, 1; k0 + 8 (t % 4) ..) with NT = 2 TN
+// threads: fetched in FP32, formed and split when staged.
+template <int NT>
+__device__ inline void load_g(const Gemm2Args &g, int c0, int k0, float4 (&ry)[4],
+                              float4 (&rx)[4]) {
   const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
-    const int r = c0 + row + 64 * h;
+    const int r = c0 + row + (NT / 4) * h;
     if (r < g.npd) {
       const float4 *py = reinterpret_cast<const float4 *>(g.YTf + (size_t)r * g.Np + k0 + kc);
       const float4 *px = reinterpret_cast<const float4 *>(g.XtTf + (size_t)r * g.Np + k0 + kc);
@@ -271,6 +293,7 @@ __device__ inline void load_g(const Gemm2Args &g, const double *c1, const double
   }
 }
 
+template <int NT>
 __device__ inline void store_g(uint16_t *lh, uint16_t *ll, const float (&a)[8], const float (&b)[8],
                                const float4 (&ry)[4], const float4 (&rx)[4]) {
   const int t = threadIdx.x, row = t >> 2, kc = (t & 3) * 8;
@@ -285,8 +308,9 @@ __device__ inline void store_g(uint16_t *lh, uint16_t *ll, const float (&a)[8], 
     for (int j = 0; j < 4; ++j)
       split2(fmaf(a[2 * j], y[2 * j], b[2 * j] * x[2 * j]),
              fmaf(a[2 * j + 1], y[2 * j + 1], b[2 * j + 1] * x[2 * j + 1]), hi[j], lo[j]);
-    *reinterpret_cast<uint4 *>(lh + (row + 64 * h) * LDK + kc) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
-    *reinterpret_cast<uint4 *>(ll + (row + 64 * h) * LDK + kc) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    const int o = (row + (NT / 4) * h) * LDK + kc;
+    *reinterpret_cast<uint4 *>(lh + o) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    *reinterpret_cast<uint4 *>(ll + o) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
   }
 }
 
@@ -300,9 +324,10 @@ __device__ inline void load_w(const double *c1, const double *c2, int k0, float 
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_gemm2_bf16x3(Gemm2Args g) {
-  constexpr int TM = TM2, WM = TM2 / 32;
-  __shared__ Stage<TM2> sm;
+template <int TM, int TN>
+__global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm2_bf16x3(Gemm2Args g) {
+  constexpr int NT = Tile<TM, TN>::NT, WC = Tile<TM, TN>::WC, WM = Tile<TM, TN>::WM;
+  __shared__ Stage<TM, TN> sm;
   const int total = g.sb * g.n_mt * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
@@ -311,7 +336,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm2_bf16x3(Gemm2Args g) {
   const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
   const int m0 = mt * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int wr = w >> 1, wc = w & 1, lr = l & 15, lg = l >> 4;
+  const int wr = w / WC, wc = w % WC, lr = l & 15, lg = l >> 4;
   const int mi = clamp_blocks(g.npd, m0 + wr * (TM / 2), WM);
   const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
   const double *c1 = g.W + (size_t)sl * 4 * g.Np, *c2 = c1 + g.Np;
@@ -323,27 +348,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm2_bf16x3(Gemm2Args g) {
     for (int b = 0; b < WN; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
   const int KT = g.Np / TK;
-  uint4 rah[TM / 64], ral[TM / 64];
+  uint4 rah[4 * TM / NT], ral[4 * TM / NT];
   float4 ry[4], rx[4];
   float wa[8], wb[8];
-  load_rows<TM>(g.Ah, g.Np, g.npd, m0, 0, rah);
-  load_rows<TM>(g.Al, g.Np, g.npd, m0, 0, ral);
-  load_g(g, c1, c2, c0, 0, ry, rx);
+  load_rows<TM, NT>(g.Ah, g.Np, g.npd, m0, 0, rah);
+  load_rows<TM, NT>(g.Al, g.Np, g.npd, m0, 0, ral);
+  load_g<NT>(g, c0, 0, ry, rx);
   load_w(c1, c2, 0, wa, wb);
   for (int kt = 0; kt < KT; ++kt) {
     __syncthreads();
-    store_rows<TM>(sm.ah, rah);
-    store_rows<TM>(sm.al, ral);
-    store_g(sm.bh, sm.bl, wa, wb, ry, rx);
+    store_rows<TM, NT>(sm.ah, rah);
+    store_rows<TM, NT>(sm.al, ral);
+    store_g<NT>(sm.bh, sm.bl, wa, wb, ry, rx);
     __syncthreads();
     if (kt + 1 < KT) {
       const int k0 = (kt + 1) * TK;
-      load_rows<TM>(g.Ah, g.Np, g.npd, m0, k0, rah);
-      load_rows<TM>(g.Al, g.Np, g.npd, m0, k0, ral);
-      load_g(g, c1, c2, c0, k0, ry, rx);
+      load_rows<TM, NT>(g.Ah, g.Np, g.npd, m0, k0, rah);
+      load_rows<TM, NT>(g.Al, g.Np, g.npd, m0, k0, ral);
+      load_g<NT>(g, c0, k0, ry, rx);
       load_w(c1, c2, k0, wa, wb);
     }
-    mma_slice<TM>(sm, wr, wc, mi, nj, acc);
+    mma_slice<TM, TN>(sm, wr, wc, mi, nj, acc);
   }
   double *Bo = g.B + (size_t)sl * g.npd * g.npd;
 #pragma unroll
@@ -360,21 +385,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm2_bf16x3(Gemm2Args g) {
       }
 }
 
-}  // namespace
-
-int bf16_col_tiles(const Ensemble &e) { return (e.npd + TN - 1) / TN; }
-
-void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
+template <int TM, int TN>
+void gemm1_launch(const Ensemble &e, int sb, int what, hipStream_t st) {
   Gemm1Args g;
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
   g.Xh = e.X_hi; g.Xl = e.X_lo;
   g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
   g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
-  g.n_it = e.Np / TM1; g.n_ct = bf16_col_tiles(e);
+  g.n_it = e.Np / TM; g.n_ct = (e.npd + TN - 1) / TN;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
-  hipLaunchKernelGGL(k_gemm1_bf16x3, dim3(8 * ((total + 7) / 8)), dim3(256), 0, st, g);
+  hipLaunchKernelGGL((k_gemm1_bf16x3<TM, TN>), dim3(8 * ((total + 7) / 8)), dim3(2 * TN), 0, st,
+                     g);
   const double kf = (what & kTermF) ? 1.0 : 0.0;
   const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
   const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
@@ -382,14 +405,34 @@ void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
                      e.W, e.Np, g.n_ct, sb, kf, kr, ky);
 }
 
-void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
+template <int TM, int TN>
+void gemm2_launch(const Ensemble &e, int sb, hipStream_t st) {
   Gemm2Args g;
   g.Ah = e.XtT_hi; g.Al = e.XtT_lo;
   g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.W = e.W; g.B = e.B;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
-  g.n_mt = (e.npd + TM2 - 1) / TM2; g.n_ct = (e.npd + TN - 1) / TN;
+  g.n_mt = (e.npd + TM - 1) / TM; g.n_ct = (e.npd + TN - 1) / TN;
   const int total = sb * g.n_mt * g.n_ct;
-  hipLaunchKernelGGL(k_gemm2_bf16x3, dim3(8 * ((total + 7) / 8)), dim3(256), 0, st, g);
+  hipLaunchKernelGGL((k_gemm2_bf16x3<TM, TN>), dim3(8 * ((total + 7) / 8)), dim3(2 * TN), 0, st,
+                     g);
+}
+
+}  // namespace
+
+int bf16_col_tiles(const Ensemble &e) { return (e.npd + kBfTileN - 1) / kBfTileN; }
+
+void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
+  if (e.bf_variant == 1)
+    gemm1_launch<256, 256>(e, sb, what, st);
+  else
+    gemm1_launch<256, 128>(e, sb, what, st);
+}
+
+void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
+  if (e.bf_variant == 1)
+    gemm2_launch<256, 256>(e, sb, st);
+  else
+    gemm2_launch<192, 128>(e, sb, st);
 }
 
 }  // namespace tdscha

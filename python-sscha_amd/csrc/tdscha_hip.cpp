@@ -603,31 +603,59 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   TD_CHECK(hipEventCreate(&t0));
   TD_CHECK(hipEventCreate(&t1));
   const int what = kTermF | kTermD2vR | kTermD2vY;
-  int best = e.s_batch;
+  int best = e.s_batch, best_v = 0;
   double best_t = 1e300;
-  for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
-    e.pass_slots = c;
-    const int ns = c / n_chains;
-    const int s_end = (2 * ns < e.N_sym) ? 2 * ns : e.N_sym;   // two passes (or all images)
-    rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);   // warm-up
-    if (!rc && hipEventRecord(t0, nullptr) != hipSuccess) rc = 10;
-    if (!rc) rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);
-    if (!rc && hipEventRecord(t1, nullptr) != hipSuccess) rc = 10;
-    float ms = 0.f;
-    if (!rc && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
-      rc = 10;
-    if (rc) break;
-    const double per = 1e3 * ms / ((double)s_end * n_chains * e.N);
-    if (per < best_t) { best_t = per; best = c; }
-    if (s_end == e.N_sym && c < e.s_batch) break;   // smaller passes cannot fill more
+  const int n_variants = precision == kBF16x3 ? 2 : 1;
+  for (int v = 0; v < n_variants && !rc; ++v) {
+    e.bf_variant = v;
+    for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
+      e.pass_slots = c;
+      const int ns = c / n_chains;
+      const int s_end = (2 * ns < e.N_sym) ? 2 * ns : e.N_sym;   // two passes (or all images)
+      rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);   // warm-up
+      if (!rc && hipEventRecord(t0, nullptr) != hipSuccess) rc = 10;
+      if (!rc) rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);
+      if (!rc && hipEventRecord(t1, nullptr) != hipSuccess) rc = 10;
+      float ms = 0.f;
+      if (!rc && (hipEventSynchronize(t1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+        rc = 10;
+      if (rc) break;
+      const double per = 1e3 * ms / ((double)s_end * n_chains * e.N);
+      if (per < best_t) { best_t = per; best = c; best_v = v; }
+      if (s_end == e.N_sym && c < e.s_batch) break;   // smaller passes cannot fill more
+    }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   g_prof.on = prof;
-  if (rc) { e.pass_slots = 0; if (rc == 10) set_error("autotune timing failed"); return rc; }
+  if (rc) {
+    e.pass_slots = 0;
+    e.bf_variant = 0;
+    if (rc == 10) set_error("autotune timing failed");
+    return rc;
+  }
   e.pass_slots = best;
+  e.bf_variant = best_v;
   if (slots_out) *slots_out = best;
   if (us_per_image) *us_per_image = best_t;
+  return 0;
+}
+
+int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 1 || pass_slots < 0) {
+    set_error("bad argument");
+    return 1;
+  }
+  h->e.bf_variant = bf16_tiles;
+  h->e.pass_slots = pass_slots;
+  return 0;
+}
+
+int tdscha_ensemble_get_tuning(const tdscha_ensemble *h, int *bf16_tiles, int *pass_slots) {
+  if (!h || !bf16_tiles || !pass_slots) { set_error("NULL argument"); return 1; }
+  *bf16_tiles = h->e.bf_variant;
+  *pass_slots = h->e.pass_slots > 0 ? h->e.pass_slots : h->e.s_batch;
   return 0;
 }
 

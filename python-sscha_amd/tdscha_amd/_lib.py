@@ -49,6 +49,8 @@ SIGNATURES = {
         _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
     "tdscha_ensemble_reserve": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tdscha_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip, _dp]),
+    "tdscha_ensemble_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "tdscha_ensemble_get_tuning": (ctypes.c_int, [_vp, _ip, _ip]),
     "tdscha_ensemble_force_grad": (ctypes.c_int, [
         _vp, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, ctypes.c_int,
         _vp]),

@@ -183,6 +183,18 @@ class DeviceEnsemble:
                 "tdscha_ensemble_autotune")
         return slots.value, per.value
 
+    def set_tuning(self, bf16_tiles=0, pass_slots=0):
+        """Contraction tile shape (0: 256/192 x 128, 1: 256 x 256) and image slots per pass
+        (0 = all) as the autotuner would set them (tdscha_ensemble_set_tuning)."""
+        L.check(L.lib().tdscha_ensemble_set_tuning(self._h, int(bf16_tiles), int(pass_slots)),
+                "tdscha_ensemble_set_tuning")
+
+    def tuning(self):
+        v, s = ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().tdscha_ensemble_get_tuning(self._h, ctypes.byref(v), ctypes.byref(s)),
+                "tdscha_ensemble_get_tuning")
+        return v.value, s.value
+
     def force_grad(self, us, vs, flags=None, out=None, accumulate=False):
         """d/dY of sum_p u_p . A_p(v_p) (tdscha_ensemble_force_grad): A_p is the anharmonic
         part of the operator with flags[p]. Returns an (N, n) FP64 device tensor."""
