@@ -385,6 +385,270 @@ __global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm2_bf16x3(Gemm2Args g) 
       }
 }
 
+// ---------------------------------------------------- variant 2: direct global->LDS loads
+// 256 x 256 tiles, 8 waves, two LDS stages of unpadded 64-byte rows (one k slice each). The
+// operand planes are copied straight into the next stage with global_load_lds_dwordx4
+// while the current stage is multiplied, so a slice costs one barrier and no register
+// staging. A wave's copy instruction fills 1 KB = 16 rows contiguously (lane l -> row l/4,
+// 16-byte slot l%4); the k chunk stored in slot j of row r is j ^ sw(r), sw(r) = (r >> 2) & 3,
+// so the 16 rows read by one fragment load hit 16 distinct bank groups.
+constexpr int DT = 256;                 // tile rows and columns
+constexpr int DNT = 512;                // threads
+constexpr int DROW = TK;                // bf16 per LDS row (64 bytes)
+
+struct DStage {
+  uint16_t ah[DT * DROW], al[DT * DROW], bh[DT * DROW], bl[DT * DROW];   // 64 KB
+};
+
+__device__ inline int swz(int row) { return (row >> 2) & 3; }
+
+__device__ inline v8bf dfrag(const uint16_t *plane, int row, int kg) {
+  return __builtin_bit_cast(
+      v8bf, *reinterpret_cast<const uint4 *>(plane + row * DROW + ((kg ^ swz(row)) * 8)));
+}
+
+// This wave's share of one 256-row plane slice: groups w and w + 8 of 16 rows. Rows past
+// `rows` read row rows - 1 instead (their products are never stored).
+__device__ inline void dma_plane(const uint16_t *__restrict__ P, size_t ld, int rows, int r0,
+                                 int k0, uint16_t *lds_plane) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int grp = w + 8 * h;
+    const int row = grp * 16 + (l >> 2);
+    int r = r0 + row;
+    r = r < rows ? r : rows - 1;
+    const uint16_t *src = P + (size_t)r * ld + k0 + (((l & 3) ^ swz(row)) * 8);
+    __builtin_amdgcn_global_load_lds(
+        (const void __attribute__((address_space(1))) *)src,
+        (void __attribute__((address_space(3))) *)(lds_plane + grp * 16 * DROW), 16, 0, 0);
+  }
+}
+
+__device__ inline void dma_mma(const DStage &s, int wr, int wc, v4f (&acc)[8][WN]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  v8bf bh[WN], bl[WN];
+#pragma unroll
+  for (int tj = 0; tj < WN; ++tj) {
+    bh[tj] = dfrag(s.bh, wc * 64 + tj * 16 + lr, lg);
+    bl[tj] = dfrag(s.bl, wc * 64 + tj * 16 + lr, lg);
+  }
+#pragma unroll
+  for (int ti = 0; ti < 8; ++ti) {
+    const v8bf ah = dfrag(s.ah, wr * 128 + ti * 16 + lr, lg);
+    const v8bf al = dfrag(s.al, wr * 128 + ti * 16 + lr, lg);
+#pragma unroll
+    for (int tj = 0; tj < WN; ++tj) {
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[tj], acc[ti][tj], 0, 0, 0);
+      acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[tj], acc[ti][tj], 0, 0, 0);
+    }
+  }
+}
+
+// every copy into LDS issued by this wave has landed, then all waves meet
+__device__ inline void dma_sync() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(DNT, 1) void k_gemm1_bf16x3_dma(Gemm1Args g) {
+  __shared__ DStage st[2];
+  const int total = g.sb * g.n_it * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id / (g.n_it * g.n_ct);
+  const int rest = id - sl * g.n_it * g.n_ct;
+  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  const int i0 = it * DT, c0 = ct * DT;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 2, wc = w & 3, lr = l & 15, lg = l >> 4;
+  const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
+  const size_t plane_off = (size_t)sl * g.npd * g.npd;
+  const uint16_t *Yh = g.Yh + plane_off, *Yl = g.Yl + plane_off;
+
+  v4f acc[8][WN];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  if (g.do_q) {
+    const int KT = g.npd / TK;
+    dma_plane(g.Xh, g.npd, g.Np, i0, 0, st[0].ah);
+    dma_plane(g.Xl, g.npd, g.Np, i0, 0, st[0].al);
+    dma_plane(Yh, g.npd, g.npd, c0, 0, st[0].bh);
+    dma_plane(Yl, g.npd, g.npd, c0, 0, st[0].bl);
+    dma_sync();
+    for (int kt = 0; kt < KT; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < KT) {
+        const int k0 = (kt + 1) * TK;
+        dma_plane(g.Xh, g.npd, g.Np, i0, k0, st[cur ^ 1].ah);
+        dma_plane(g.Xl, g.npd, g.Np, i0, k0, st[cur ^ 1].al);
+        dma_plane(Yh, g.npd, g.npd, c0, k0, st[cur ^ 1].bh);
+        dma_plane(Yl, g.npd, g.npd, c0, k0, st[cur ^ 1].bl);
+      }
+      if (nj > 0) dma_mma(st[cur], wr, wc, acc);
+      dma_sync();
+    }
+  }
+
+  // epilogue as in k_gemm1_bf16x3, the row sums staged in the (now free) LDS stages
+  double (*red)[DT][4] = reinterpret_cast<double (*)[DT][4]>(&st[0]);   // [4][256][4]: 32 KB
+  const double *g1 = g.G + (size_t)sl * 2 * g.npd, *g2 = g1 + g.npd;
+#pragma unroll
+  for (int ti = 0; ti < 8; ++ti) {
+    double p[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
+#pragma unroll
+    for (int tj = 0; tj < WN; ++tj) {
+      const int c = c0 + wc * 64 + tj * 16 + lr;
+      if (tj < nj) {
+        const double gg1 = g1[c], gg2 = g2[c];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = wr * 128 + ti * 16 + 4 * lg + r;
+          const size_t o = (size_t)(i0 + m) * g.npd + c;
+          const double x = g.Xf[o], ft = g.Ftf[o];
+          const double q = (double)acc[ti][tj][r];
+          p[r][0] = fma(x, q, p[r][0]);
+          p[r][1] = fma(ft, q, p[r][1]);
+          p[r][2] = fma(x, gg1, p[r][2]);
+          p[r][3] = fma(ft, gg2, p[r][3]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double v = p[r][q];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+        if (lr == 0) red[wc][wr * 128 + ti * 16 + 4 * lg + r][q] = v;
+      }
+  }
+  __syncthreads();
+  double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np + i0;
+  for (int idx = threadIdx.x; idx < DT * 4; idx += blockDim.x) {
+    const int q = idx / DT, m = idx - q * DT;
+    Pt[(size_t)q * g.Np + m] = red[0][m][q] + red[1][m][q] + red[2][m][q] + red[3][m][q];
+  }
+}
+
+// B operand (G^T, formed) of a 256-column tile with 512 threads: rows c0 + t/4 + 128 h,
+// stored into the swizzled layout.
+__device__ inline void dstore_g(uint16_t *lh, uint16_t *ll, const float (&a)[8],
+                                const float (&b)[8], const float4 (&ry)[4],
+                                const float4 (&rx)[4]) {
+  const int t = threadIdx.x, row0 = t >> 2, j = t & 3;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const float y[8] = {ry[2 * h].x, ry[2 * h].y, ry[2 * h].z, ry[2 * h].w,
+                        ry[2 * h + 1].x, ry[2 * h + 1].y, ry[2 * h + 1].z, ry[2 * h + 1].w};
+    const float x[8] = {rx[2 * h].x, rx[2 * h].y, rx[2 * h].z, rx[2 * h].w,
+                        rx[2 * h + 1].x, rx[2 * h + 1].y, rx[2 * h + 1].z, rx[2 * h + 1].w};
+    uint32_t hi[4], lo[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      split2(fmaf(a[2 * q], y[2 * q], b[2 * q] * x[2 * q]),
+             fmaf(a[2 * q + 1], y[2 * q + 1], b[2 * q + 1] * x[2 * q + 1]), hi[q], lo[q]);
+    const int row = row0 + 128 * h;
+    const int o = row * DROW + ((j ^ swz(row)) * 8);
+    *reinterpret_cast<uint4 *>(lh + o) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    *reinterpret_cast<uint4 *>(ll + o) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+  }
+}
+
+__global__ __launch_bounds__(DNT, 1) void k_gemm2_bf16x3_dma(Gemm2Args g) {
+  __shared__ DStage st[2];
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id / (g.n_mt * g.n_ct);
+  const int rest = id - sl * g.n_mt * g.n_ct;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int m0 = mt * DT, c0 = ct * DT;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = w >> 2, wc = w & 3, lr = l & 15, lg = l >> 4;
+  const int mi = clamp_blocks(g.npd, m0 + wr * 128, 8);
+  const int nj = clamp_blocks(g.npd, c0 + wc * 64, WN);
+  const double *c1 = g.W + (size_t)sl * 4 * g.Np, *c2 = c1 + g.Np;
+
+  v4f acc[8][WN];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int KT = g.Np / TK;
+  float4 ry[4], rx[4];
+  float wa[8], wb[8];
+  dma_plane(g.Ah, g.Np, g.npd, m0, 0, st[0].ah);
+  dma_plane(g.Al, g.Np, g.npd, m0, 0, st[0].al);
+  load_g<DNT>(g, c0, 0, ry, rx);
+  load_w(c1, c2, 0, wa, wb);
+  dstore_g(st[0].bh, st[0].bl, wa, wb, ry, rx);
+  dma_sync();
+  for (int kt = 0; kt < KT; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < KT;
+    if (more) {
+      const int k0 = (kt + 1) * TK;
+      dma_plane(g.Ah, g.Np, g.npd, m0, k0, st[cur ^ 1].ah);
+      dma_plane(g.Al, g.Np, g.npd, m0, k0, st[cur ^ 1].al);
+      load_g<DNT>(g, c0, k0, ry, rx);
+      load_w(c1, c2, k0, wa, wb);
+    }
+    if (mi > 0 && nj > 0) dma_mma(st[cur], wr, wc, acc);
+    if (more) dstore_g(st[cur ^ 1].bh, st[cur ^ 1].bl, wa, wb, ry, rx);
+    dma_sync();
+  }
+  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
+#pragma unroll
+  for (int ti = 0; ti < 8; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < WN; ++tj)
+      if (ti < mi && tj < nj) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wr * 128 + ti * 16 + 4 * lg + r;
+          const int c = c0 + wc * 64 + tj * 16 + lr;
+          Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
+        }
+      }
+}
+
+void gemm1_launch_dma(const Ensemble &e, int sb, int what, hipStream_t st) {
+  Gemm1Args g;
+  const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+  g.Xh = e.X_hi; g.Xl = e.X_lo;
+  g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
+  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_it = e.Np / DT; g.n_ct = (e.npd + DT - 1) / DT;
+  g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  const int total = sb * g.n_it * g.n_ct;
+  hipLaunchKernelGGL(k_gemm1_bf16x3_dma, dim3(8 * ((total + 7) / 8)), dim3(DNT), 0, st, g);
+  const double kf = (what & kTermF) ? 1.0 : 0.0;
+  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
+  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
+  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
+                     e.W, e.Np, g.n_ct, sb, kf, kr, ky);
+}
+
+void gemm2_launch_dma(const Ensemble &e, int sb, hipStream_t st) {
+  Gemm2Args g;
+  g.Ah = e.XtT_hi; g.Al = e.XtT_lo;
+  g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.W = e.W; g.B = e.B;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_mt = (e.npd + DT - 1) / DT; g.n_ct = (e.npd + DT - 1) / DT;
+  const int total = sb * g.n_mt * g.n_ct;
+  hipLaunchKernelGGL(k_gemm2_bf16x3_dma, dim3(8 * ((total + 7) / 8)), dim3(DNT), 0, st, g);
+}
+
 template <int TM, int TN>
 void gemm1_launch(const Ensemble &e, int sb, int what, hipStream_t st) {
   Gemm1Args g;
@@ -422,14 +686,18 @@ void gemm2_launch(const Ensemble &e, int sb, hipStream_t st) {
 int bf16_col_tiles(const Ensemble &e) { return (e.npd + kBfTileN - 1) / kBfTileN; }
 
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
-  if (e.bf_variant == 1)
+  if (e.bf_variant == 2)
+    gemm1_launch_dma(e, sb, what, st);
+  else if (e.bf_variant == 1)
     gemm1_launch<256, 256>(e, sb, what, st);
   else
     gemm1_launch<256, 128>(e, sb, what, st);
 }
 
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
-  if (e.bf_variant == 1)
+  if (e.bf_variant == 2)
+    gemm2_launch_dma(e, sb, st);
+  else if (e.bf_variant == 1)
     gemm2_launch<256, 256>(e, sb, st);
   else
     gemm2_launch<192, 128>(e, sb, st);

@@ -605,7 +605,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   const int what = kTermF | kTermD2vR | kTermD2vY;
   int best = e.s_batch, best_v = 0;
   double best_t = 1e300;
-  const int n_variants = precision == kBF16x3 ? 2 : 1;
+  const int n_variants = precision == kBF16x3 ? 3 : 1;
   for (int v = 0; v < n_variants && !rc; ++v) {
     e.bf_variant = v;
     for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
@@ -643,7 +643,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 1 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 2 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

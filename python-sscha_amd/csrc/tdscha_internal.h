@@ -113,7 +113,8 @@ struct Ensemble {
   // their row sums and the gradient accumulator
   int adj_chains = 0, adj_slots = 0;
   int pass_slots = 0;         // slots used per pass (autotuned; 0 = s_batch)
-  int bf_variant = 0;         // BF16x3 tile shapes (autotuned): 0 = 256/192 x 128, 1 = 256 x 256
+  int bf_variant = 0;         // BF16x3 contraction kernels (autotuned): 0 = 256/192 x 128,
+                              // 1 = 256 x 256, 2 = 256 x 256 with direct global->LDS copies
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups

@@ -184,8 +184,9 @@ class DeviceEnsemble:
         return slots.value, per.value
 
     def set_tuning(self, bf16_tiles=0, pass_slots=0):
-        """Contraction tile shape (0: 256/192 x 128, 1: 256 x 256) and image slots per pass
-        (0 = all) as the autotuner would set them (tdscha_ensemble_set_tuning)."""
+        """BF16x3 contraction kernels (0: 256/192 x 128 tiles, 1: 256 x 256, 2: 256 x 256 with
+        direct global->LDS copies) and image slots per pass (0 = all), as the autotuner sets
+        them (tdscha_ensemble_set_tuning)."""
         L.check(L.lib().tdscha_ensemble_set_tuning(self._h, int(bf16_tiles), int(pass_slots)),
                 "tdscha_ensemble_set_tuning")
 

@@ -159,3 +159,20 @@ def test_autotuned_pass_size_keeps_results():
     after = [dev.apply_full_L(psi, precision=p) for p in ("fp64", "bf16x3")]
     for x, y in zip(before, after):
         assert torch.equal(x, y)
+
+
+def test_batched_driver_small():
+    """The multi-system driver on two small systems: 12 starting vectors each (IR, Raman-like,
+    random), autotuned passes, one graph per step."""
+    import torch
+    from tdscha_amd import batched
+    group, members, ensembles, info = batched.build(systems=[(2, 300), (2, 512)],
+                                                    precision="bf16x3", max_steps=4)
+    group.run(3)
+    torch.cuda.synchronize()
+    assert [i["chains"] for i in info] == [12, 12]
+    for m in members:
+        for ch in range(m.nc):
+            a, b, c, _ = m.coefficients(ch, truncated=False)
+            assert len(a) == 3 and np.all(np.isfinite(a)) and np.all(np.isfinite(b))
+    group.close()
