@@ -162,15 +162,16 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
 
 /* Autotuner (no reference counterpart): times two passes of the batched L-application for
  * n_chains chains at slot counts s_batch, s_batch/2, ... (at most six) and, for BF16x3, the
- * three contraction kernel forms, and keeps the fastest per image for this ensemble. Returns the chosen slot count and its
- * time per (chain x configuration x symmetry) image in microseconds (either may be NULL).
- * Call before capturing Lanczos graphs on the ensemble. */
+ * four contraction kernel forms, and keeps the fastest per image for this ensemble. Returns
+ * the chosen slot count and its time per (chain x configuration x symmetry) image in
+ * microseconds (either may be NULL). Call before capturing Lanczos graphs on the ensemble. */
 int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, int *slots,
                              double *us_per_image);
 
 /* The tuning the autotuner sets, set or read directly: bf16_tiles 0 (256/192 x 128
- * contraction tiles), 1 (256 x 256) or 2 (256 x 256, operands copied straight into
- * double-buffered LDS); pass_slots image slots per pass (0 = all s_batch).
+ * contraction tiles), 1 (256 x 256), 2 (256 x 256, operands copied straight into
+ * double-buffered LDS) or 3 (256/192 x 192, A operands in registers, only B through LDS);
+ * pass_slots image slots per pass (0 = all s_batch).
  * Results do not depend on pass_slots; the tile shape only changes the order of the FP64
  * row sums of GEMM1. */
 int tdscha_ensemble_set_tuning(tdscha_ensemble *ens, int bf16_tiles, int pass_slots);

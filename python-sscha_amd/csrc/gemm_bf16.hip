@@ -25,17 +25,14 @@
 // GEMM2 (per image s):  B_s = X~^T G_s with G_s^T[c][k] = c1_k Y^T[c][k] + c2_k/2 X~^T[c][k]
 //   formed in FP32 from the FP32 operand copies and split while it is staged.
 #include "tdscha_internal.h"
+#include "bf16x3.h"
 
 namespace tdscha {
 
+using namespace bfx;
+
 namespace {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef float v2f __attribute__((ext_vector_type(2)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
-
-constexpr int TK = 32;           // k slice
 constexpr int LDK = TK + 8;      // LDS row stride in bf16
 constexpr int WN = 4;            // 16 x 16 column blocks per wave (64 columns)
 
@@ -51,23 +48,6 @@ template <int TM, int TN> struct Tile {
 template <int TM, int TN> struct Stage {
   uint16_t ah[TM * LDK], al[TM * LDK], bh[TN * LDK], bl[TN * LDK];
 };
-
-// Workgroup id -> logical tile id, so that XCD x runs logical ids [x * per, (x + 1) * per).
-// The grid has 8 * per workgroups; ids past `total` return.
-__device__ inline int xcd_tile(int total) {
-  const int per = (total + 7) / 8;
-  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-}
-
-// Pairs of FP32 values split into packed bf16 hi / lo (v_cvt_pk_bf16_f32, nearest even).
-__device__ inline void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
-  const v2f f = {a, b};
-  const v2bf h = __builtin_convertvector(f, v2bf);
-  const v2f hf = __builtin_convertvector(h, v2f);
-  const v2bf l = __builtin_convertvector(f - hf, v2bf);
-  hi = __builtin_bit_cast(uint32_t, h);
-  lo = __builtin_bit_cast(uint32_t, l);
-}
 
 // R rows x 32 k of one bf16 plane with NT threads: 16-byte chunks, rows t/4 + (NT/4) h
 // (zero past `rows`).
@@ -128,12 +108,6 @@ template <int TM, int TN>
 __device__ inline void mma_slice(const Stage<TM, TN> &s, int wr, int wc, int mi, int nj,
                                  v4f (&acc)[Tile<TM, TN>::WM][WN]) {
   if (mi > 0 && nj > 0) mma_full<TM, TN>(s, wr, wc, acc);
-}
-
-__device__ inline int clamp_blocks(int valid, int start, int cap) {
-  int b = (valid - start + 15) / 16;
-  b = b < 0 ? 0 : (b > cap ? cap : b);
-  return __builtin_amdgcn_readfirstlane(b);
 }
 
 // ------------------------------------------------------------------------------ GEMM1
@@ -243,9 +217,11 @@ __global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm1_bf16x3(Gemm1Args g) 
 }
 
 // W[s] = (c1, c2, e1, e2) from the column-tile row sums, tiles added in increasing order.
+// Wf (when not null) receives c1 and c2 / 2 in FP32, the weights as the GEMM2 operand
+// formation uses them.
 __global__ void k_weights_bf16(const double *__restrict__ P, const double *__restrict__ rho,
-                               double *__restrict__ W, int Np, int n_ct, int sb, double kf,
-                               double kr, double ky) {
+                               double *__restrict__ W, float *__restrict__ Wf, int Np, int n_ct,
+                               int sb, double kf, double kr, double ky) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = blockIdx.y;
   if (i >= Np) return;
@@ -256,10 +232,17 @@ __global__ void k_weights_bf16(const double *__restrict__ P, const double *__res
     for (int q = 0; q < 4; ++q) s[q] += Ps[((size_t)ct * 4 + q) * Np];
   const double r = rho[i];
   double *Ws = W + (size_t)sl * 4 * Np;
-  Ws[i] = r * (ky * s[0] / 8 - kr * s[2] / 3);
-  Ws[(size_t)Np + i] = r * (ky * s[1] / 4 - kr * s[3] / 3);
+  const double c1 = r * (ky * s[0] / 8 - kr * s[2] / 3);
+  const double c2 = r * (ky * s[1] / 4 - kr * s[3] / 3);
+  Ws[i] = c1;
+  Ws[(size_t)Np + i] = c2;
   Ws[2 * (size_t)Np + i] = -kf * r * s[0] / 6;
   Ws[3 * (size_t)Np + i] = -kf * r * s[1] / 3;
+  if (Wf) {
+    float *Wfs = Wf + (size_t)sl * 2 * Np;
+    Wfs[i] = (float)c1;
+    Wfs[(size_t)Np + i] = (float)(0.5 * c2);
+  }
 }
 
 // ------------------------------------------------------------------------------ GEMM2
@@ -632,11 +615,7 @@ void gemm1_launch_dma(const Ensemble &e, int sb, int what, hipStream_t st) {
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL(k_gemm1_bf16x3_dma, dim3(8 * ((total + 7) / 8)), dim3(DNT), 0, st, g);
-  const double kf = (what & kTermF) ? 1.0 : 0.0;
-  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
-  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
-  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
-                     e.W, e.Np, g.n_ct, sb, kf, kr, ky);
+  launch_weights_bf16(e, sb, g.n_ct, what, st);
 }
 
 void gemm2_launch_dma(const Ensemble &e, int sb, hipStream_t st) {
@@ -662,11 +641,7 @@ void gemm1_launch(const Ensemble &e, int sb, int what, hipStream_t st) {
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL((k_gemm1_bf16x3<TM, TN>), dim3(8 * ((total + 7) / 8)), dim3(2 * TN), 0, st,
                      g);
-  const double kf = (what & kTermF) ? 1.0 : 0.0;
-  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
-  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
-  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
-                     e.W, e.Np, g.n_ct, sb, kf, kr, ky);
+  launch_weights_bf16(e, sb, g.n_ct, what, st);
 }
 
 template <int TM, int TN>
@@ -685,8 +660,18 @@ void gemm2_launch(const Ensemble &e, int sb, hipStream_t st) {
 
 int bf16_col_tiles(const Ensemble &e) { return (e.npd + kBfTileN - 1) / kBfTileN; }
 
+void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStream_t st) {
+  const double kf = (what & kTermF) ? 1.0 : 0.0;
+  const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
+  const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
+  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
+                     e.W, e.Wf, e.Np, n_ct, sb, kf, kr, ky);
+}
+
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
-  if (e.bf_variant == 2)
+  if (e.bf_variant == 3)
+    launch_gemm1_bf16x3_rp(e, sb, what, st);
+  else if (e.bf_variant == 2)
     gemm1_launch_dma(e, sb, what, st);
   else if (e.bf_variant == 1)
     gemm1_launch<256, 256>(e, sb, what, st);
@@ -695,7 +680,9 @@ void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
 }
 
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
-  if (e.bf_variant == 2)
+  if (e.bf_variant == 3)
+    launch_gemm2_bf16x3_rp(e, sb, st);
+  else if (e.bf_variant == 2)
     gemm2_launch_dma(e, sb, st);
   else if (e.bf_variant == 1)
     gemm2_launch<256, 256>(e, sb, st);

@@ -1,0 +1,473 @@
+// BF16x3 contractions with register-resident A operands (gfx950): tile form 3 of the
+// autotuner.
+//
+// A workgroup tile is TM x 192 with 512 threads = 8 waves, 4 row waves x 2 column waves.
+// Wave (wr, wc) owns rows [wr WR, (wr + 1) WR) and columns [96 wc, 96 wc + 96): WR / 16 x 6
+// blocks of 16 x 16, FP32 accumulators in VGPRs (two waves per SIMD). A fragments go from
+// global memory straight into registers in the MFMA operand layout and never touch LDS (the
+// two column waves of a row load the same rows; the second load hits L2). Only the B slice
+// (192 rows x 32 k, hi and lo planes, 24 KB) is shared through LDS: two stages of unpadded
+// 64-byte rows, 16-byte chunk j of row r stored at chunk j ^ sw(r),
+// sw(r) = ((r >> 1) ^ (r >> 3)) & 3, so that the fragment reads (16 rows, one chunk each,
+// per 16 lanes) and the staging writes (4 rows x 4 chunks, or 2 rows x 8 half chunks, per 16
+// lanes) are free of bank conflicts for both 8- and 16-lane LDS phases.
+//
+// One barrier per k slice. While slice k is multiplied, B slice k + 1 (in registers since
+// the previous slice) is written to the other stage halfway through, and B slice k + 2 and A
+// slice k + 2 are in flight. The loop is unrolled by two so that every register buffer has a
+// fixed name (both k ranges hold an even number of slices), loads past the last slice re-read
+// the last slice instead of branching, and fragment reads are software-pipelined one column
+// block ahead of the MFMAs with scheduling fences that keep two blocks of fragments live. Per
+// flop, LDS traffic is about half that of tile forms 0-2 (no A staging, no A fragment reads),
+// which the counters showed to be their limit.
+//
+// GEMM1 (WR = 64, TM = 256): Q = X Ys_s with the FP64 row sums of the tile (alpha, beta,
+//   gamma, delta) as k_gemm1_bf16x3; the two column waves of a row add theirs through LDS.
+// GEMM2 (WR = 48, TM = 192): B_s = X~^T G_s with the B operand
+//   G_s^T[c][k] = c1_k Y^T[c][k] + c2_k/2 X~^T[c][k] formed once per workgroup from the FP32
+//   operand copies while it is staged, with the arithmetic of k_gemm2_bf16x3.
+#include "tdscha_internal.h"
+#include "bf16x3.h"
+
+namespace tdscha {
+
+using namespace bfx;
+
+namespace {
+
+constexpr int RTN = 192;          // tile columns
+constexpr int RWC = 96;           // columns per wave
+constexpr int RNB = RWC / 16;     // 16-column blocks per wave
+constexpr int RNT = 512;          // threads: 4 row waves x 2 column waves
+
+// Native vector registers (HIP's uint4 / float4 structs are copied with memcpy, which keeps
+// them in scratch memory).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+struct RStage {
+  uint16_t bh[RTN * TK], bl[RTN * TK];   // 2 x 12 KB
+};
+
+__device__ inline int rswz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+// ---- B slice from bf16 planes: waves 0-3 stage the hi plane, waves 4-7 the lo plane, each
+// thread rows (t % 256) / 4 + 64 h (h < 3), 16-byte chunk t % 4.
+struct BSlice {
+  u32x4 c[3];
+};
+
+__device__ inline void rload_b(const uint16_t *__restrict__ Ph, const uint16_t *__restrict__ Pl,
+                               size_t ld, int rows, int r0, int k0, BSlice &b) {
+  const int t = threadIdx.x, tt = t & 255;
+  const uint16_t *P = t < 256 ? Ph : Pl;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    int r = r0 + (tt >> 2) + 64 * h;
+    r = r < rows ? r : rows - 1;   // rows past the edge feed columns that are never used
+    b.c[h] = *reinterpret_cast<const u32x4 *>(P + (size_t)r * ld + k0 + (tt & 3) * 8);
+  }
+}
+
+__device__ inline void rstore_b(RStage &s, const BSlice &b) {
+  const int t = threadIdx.x, tt = t & 255;
+  uint16_t *dst = t < 256 ? s.bh : s.bl;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    const int row = (tt >> 2) + 64 * h;
+    *reinterpret_cast<u32x4 *>(dst + row * TK + (((tt & 3) ^ rswz(row)) * 8)) = b.c[h];
+  }
+}
+
+// ---- A operand from fragment-tiled bf16 planes (k_tile_bf16x3): tile (rb, ks) holds rows
+// 16 rb .. + 16 and k 32 ks .. + 32 as 64 lane chunks of 8 values in the MFMA operand layout
+// (lane l: row 16 rb + l % 16, k = 32 ks + 8 (l / 16) .. + 8), so a wave's load of one block
+// is 1 KB of consecutive 16-byte lane chunks, which the address unit coalesces (the same
+// rows read from the row-major planes cost one address cycle per lane). Row blocks past
+// n_rb read the last block (their products are never stored).
+template <int MB> struct AFrag {
+  u32x4 h[MB], l[MB];
+};
+
+template <int MB>
+__device__ inline void rload_a(const uint16_t *__restrict__ Ph, const uint16_t *__restrict__ Pl,
+                               int n_ks, int n_rb, int rb0, int ks, AFrag<MB> &a) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int ti = 0; ti < MB; ++ti) {
+    int rb = rb0 + ti;
+    rb = rb < n_rb ? rb : n_rb - 1;
+    const size_t o = (((size_t)rb * n_ks + ks) * 64 + l) * 8;
+    a.h[ti] = *reinterpret_cast<const u32x4 *>(Ph + o);
+    a.l[ti] = *reinterpret_cast<const u32x4 *>(Pl + o);
+  }
+}
+
+// fp64 [rows x cols] row-major -> fragment-tiled bf16 hi plane at dst and lo plane at
+// dst + rows * cols (rows % 16 == 0, cols % 32 == 0), with k_convert's split.
+__global__ void k_tile_bf16x3(const double *__restrict__ src, int rows, int cols,
+                              uint16_t *__restrict__ dst) {
+  const int c8 = cols / 8;
+  const size_t n8 = (size_t)rows * c8, plane = (size_t)rows * cols;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n8; i += stride) {
+    const int row = (int)(i / c8), k = (int)(i % c8) * 8;
+    const int lane = (row & 15) + 16 * ((k & 31) >> 3);
+    const size_t o = (((size_t)(row >> 4) * (cols >> 5) + (k >> 5)) * 64 + lane) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float f = (float)src[(size_t)row * cols + k + j];
+      const uint16_t h = bf16_rne(f);
+      dst[o + j] = h;
+      dst[plane + o + j] = bf16_rne(f - __uint_as_float((uint32_t)h << 16));
+    }
+  }
+}
+
+// ---- fragments and MFMAs
+struct BFrag {
+  v8bf h, l;
+};
+
+// B fragment of column block TJ of this wave: row 96 wc + 16 TJ + lr, k group lg, stored in
+// chunk lg ^ sw(lr) ^ 2 (TJ & 1) of that row (96 wc is a multiple of 32). `base` holds the
+// offsets of the two parities for this lane.
+template <int TJ>
+__device__ inline BFrag rfrag(const RStage &s, int2 base) {
+  const int o = TJ * 16 * TK + ((TJ & 1) ? base.y : base.x);
+  return {__builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(s.bh + o)),
+          __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(s.bl + o))};
+}
+
+__device__ inline int2 rfrag_base() {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const int row = (w >> 2) * RWC + lr;
+  return make_int2(row * TK + ((lg ^ rswz(lr)) * 8), row * TK + ((lg ^ rswz(lr) ^ 2) * 8));
+}
+
+// 3 MFMAs per 16 x 16 block of column block TJ (a_lo b_hi, a_hi b_lo, a_hi b_hi).
+template <int MB, int TJ>
+__device__ inline void rmma_block(const AFrag<MB> &a, const BFrag &b, v4f (&acc)[MB][RNB]) {
+#pragma unroll
+  for (int ti = 0; ti < MB; ++ti) {
+    const v8bf ah = __builtin_bit_cast(v8bf, a.h[ti]);
+    const v8bf al = __builtin_bit_cast(v8bf, a.l[ti]);
+    acc[ti][TJ] = mfma_bf16(al, b.h, acc[ti][TJ]);
+    acc[ti][TJ] = mfma_bf16(ah, b.l, acc[ti][TJ]);
+    acc[ti][TJ] = mfma_bf16(ah, b.h, acc[ti][TJ]);
+  }
+}
+
+// Column blocks [TJ, J1), software-pipelined: the fragments of block TJ + 1 are read before
+// the MFMAs of block TJ are issued (the group barriers pin that order: left alone, the
+// scheduler sinks the reads below most of the MFMAs that should cover their latency), and a
+// scheduling fence after each block keeps further reads from being hoisted.
+template <int MB, int TJ, int J1>
+__device__ inline void rmma_from(const RStage &s, const AFrag<MB> &a, const BFrag &cur,
+                                 int2 base, v4f (&acc)[MB][RNB]) {
+  if constexpr (TJ + 1 < J1) {
+    const BFrag nxt = rfrag<TJ + 1>(s, base);
+    rmma_block<MB, TJ>(a, cur, acc);
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);        // the two DS reads
+    __builtin_amdgcn_sched_group_barrier(0x008, 3 * MB, 0);   // then the MFMAs
+    __builtin_amdgcn_sched_barrier(0);
+    rmma_from<MB, TJ + 1, J1>(s, a, nxt, base, acc);
+  } else {
+    rmma_block<MB, TJ>(a, cur, acc);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int MB, int J0, int J1>
+__device__ inline void rmma(const RStage &s, const AFrag<MB> &a, int2 base,
+                            v4f (&acc)[MB][RNB]) {
+  const BFrag first = rfrag<J0>(s, base);
+  __builtin_amdgcn_sched_barrier(0);
+  rmma_from<MB, J0, J1>(s, a, first, base, acc);
+}
+
+// ------------------------------------------------------------------------------ GEMM1
+struct Gemm1RArgs {
+  const uint16_t *Xh, *Xl;    // [Np x npd] fragment-tiled
+  const uint16_t *Yh, *Yl;    // [s_batch x npd x npd] planes of the rotated Y1
+  const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
+  const double *G;            // [s_batch x 2 x npd]
+  double *P;                  // [sb x n_ct x 4 x Np] row sums per column tile
+  int Np, npd, sb, n_it, n_ct, do_q;
+};
+
+// FP64 row sums of block row ti of this wave over its columns below nj (alpha, beta, gamma,
+// delta; the first two from Q), reduced over the 16 lanes of each row group.
+template <int MB>
+__device__ inline void row_sums(const Gemm1RArgs &g, int sl, int ti, int row0, int cw, int nj,
+                                const v4f (&acc)[MB][RNB], double (&p)[4][4]) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const double *g1 = g.G + (size_t)sl * 2 * g.npd, *g2 = g1 + g.npd;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
+#pragma unroll
+  for (int tj = 0; tj < RNB; ++tj) {
+    if (tj < nj) {
+      const int c = cw + tj * 16 + lr;
+      const double gg1 = g1[c], gg2 = g2[c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const size_t o = (size_t)(row0 + ti * 16 + 4 * lg + r) * g.npd + c;
+        const double x = g.Xf[o], ft = g.Ftf[o];
+        const double q = (double)acc[ti][tj][r];
+        p[r][0] = fma(x, q, p[r][0]);
+        p[r][1] = fma(ft, q, p[r][1]);
+        p[r][2] = fma(x, gg1, p[r][2]);
+        p[r][3] = fma(ft, gg2, p[r][3]);
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double v = p[r][q];
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+      p[r][q] = v;
+    }
+}
+
+__global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
+  constexpr int MB = 4, TM = 256;
+  __shared__ RStage st[2];
+  const int total = g.sb * g.n_it * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  // slot-major, then row tile, column tile fastest
+  const int sl = id / (g.n_it * g.n_ct);
+  const int rest = id - sl * g.n_it * g.n_ct;
+  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  const int i0 = it * TM, c0 = ct * RTN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const int wr = w & 3, wc = w >> 2;
+  const int cw = c0 + wc * RWC;                 // first column of this wave
+  const int nj = clamp_blocks(g.npd, cw, RNB);
+  const int row0 = i0 + wr * 64;
+
+  v4f acc[MB][RNB];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < RNB; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  if (g.do_q) {
+    const size_t plane_off = (size_t)sl * g.npd * g.npd;
+    const uint16_t *Yh = g.Yh + plane_off, *Yl = g.Yl + plane_off;
+    const int KT = g.npd / TK, last = (KT - 1) * TK;   // KT is even (npd % 64 == 0)
+    const int2 base = rfrag_base();
+    const int n_ks = KT, n_rb = g.Np / 16, rb0 = row0 / 16;
+    BSlice b0, b1;
+    AFrag<MB> a0, a1;
+    rload_b(Yh, Yl, g.npd, g.npd, c0, 0, b0);
+    rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, 0, a0);
+    rload_b(Yh, Yl, g.npd, g.npd, c0, TK, b1);
+    rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, 1, a1);
+    rstore_b(st[0], b0);
+    __syncthreads();
+    for (int kt = 0; kt < KT; kt += 2) {
+      // stage 0: slice kt; b1: slice kt + 1; a0: slice kt; a1: slice kt + 1
+      rload_b(Yh, Yl, g.npd, g.npd, c0, min((kt + 2) * TK, last), b0);
+      rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
+      rstore_b(st[1], b1);
+      rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
+      rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
+      __syncthreads();
+      // stage 1: slice kt + 1; b0: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
+      rload_b(Yh, Yl, g.npd, g.npd, c0, min((kt + 3) * TK, last), b1);
+      rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
+      rstore_b(st[0], b0);
+      rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
+      rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
+      __syncthreads();
+    }
+  }
+
+  // epilogue: FP64 row sums over this wave's 96 columns, one 16-row block at a time (a
+  // 16-lane butterfly adds the 16 columns of a block). Column wave 1 leaves its sums in LDS
+  // (the stages are free: the loop ended on a barrier); column wave 0 adds them to its own,
+  // in that order, and writes the tile's row sums.
+  double(*red)[4] = reinterpret_cast<double(*)[4]>(&st[0]);   // [256][4]: 8 KB
+  if (wc == 1) {
+#pragma unroll
+    for (int ti = 0; ti < MB; ++ti) {
+      double p[4][4];
+      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
+      if (lr == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) red[wr * 64 + ti * 16 + 4 * lg + r][q] = p[r][q];
+    }
+  }
+  __syncthreads();
+  if (wc == 0) {
+    double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np;
+#pragma unroll
+    for (int ti = 0; ti < MB; ++ti) {
+      double p[4][4];
+      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
+      if (lr == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = wr * 64 + ti * 16 + 4 * lg + r;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) Pt[(size_t)q * g.Np + i0 + m] = p[r][q] + red[m][q];
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ GEMM2
+struct Gemm2RArgs {
+  const uint16_t *Ah, *Al;    // X~^T planes [npd x Np] fragment-tiled
+  const float *YTf, *XtTf;    // [npd x Np] sources of the formed operand
+  const float *Wf;            // [s_batch x 2 x Np]  c1, c2 / 2
+  double *B;                  // [s_batch x npd x npd]
+  int Np, npd, sb, n_mt, n_ct;
+};
+
+// Sources of the formed B slice: rows (t / 8) + 64 h (h < 3) of the tile, k = 4 (t % 8) .. + 4.
+struct GSrc {
+  v4f y[3], x[3], wa, wb;
+};
+
+__device__ inline void rload_g(const Gemm2RArgs &g, const float *__restrict__ wf, int c0, int k0,
+                               GSrc &s) {
+  const int t = threadIdx.x, kk = k0 + (t & 7) * 4;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    int r = c0 + (t >> 3) + 64 * h;
+    r = r < g.npd ? r : g.npd - 1;
+    s.y[h] = *reinterpret_cast<const v4f *>(g.YTf + (size_t)r * g.Np + kk);
+    s.x[h] = *reinterpret_cast<const v4f *>(g.XtTf + (size_t)r * g.Np + kk);
+  }
+  s.wa = *reinterpret_cast<const v4f *>(wf + kk);
+  s.wb = *reinterpret_cast<const v4f *>(wf + g.Np + kk);
+}
+
+// G^T = c1 Y^T + c2/2 X~^T in FP32, split into hi / lo and written as 8-byte halves of the
+// swizzled 16-byte chunks.
+__device__ inline void rstore_g(RStage &st, const GSrc &s) {
+  const int t = threadIdx.x, q = t & 7;
+#pragma unroll
+  for (int h = 0; h < 3; ++h) {
+    const int row = (t >> 3) + 64 * h;
+    uint32_t hi0, lo0, hi1, lo1;
+    split2(fmaf(s.wa.x, s.y[h].x, s.wb.x * s.x[h].x), fmaf(s.wa.y, s.y[h].y, s.wb.y * s.x[h].y),
+           hi0, lo0);
+    split2(fmaf(s.wa.z, s.y[h].z, s.wb.z * s.x[h].z), fmaf(s.wa.w, s.y[h].w, s.wb.w * s.x[h].w),
+           hi1, lo1);
+    const int o = row * TK + (((q >> 1) ^ rswz(row)) * 8) + (q & 1) * 4;
+    *reinterpret_cast<u32x2 *>(st.bh + o) = (u32x2){hi0, hi1};
+    *reinterpret_cast<u32x2 *>(st.bl + o) = (u32x2){lo0, lo1};
+  }
+}
+
+__global__ __launch_bounds__(RNT, 1) void k_gemm2_bf16x3_rp(Gemm2RArgs g) {
+  constexpr int MB = 3, TM = 192;
+  __shared__ RStage st[2];
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id / (g.n_mt * g.n_ct);
+  const int rest = id - sl * g.n_mt * g.n_ct;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int m0 = mt * TM, c0 = ct * RTN;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const int wr = w & 3, wc = w >> 2;
+  const int row0 = m0 + wr * 48, cw = c0 + wc * RWC;
+  const int mi = clamp_blocks(g.npd, row0, MB);
+  const int nj = clamp_blocks(g.npd, cw, RNB);
+  const float *wf = g.Wf + (size_t)sl * 2 * g.Np;
+
+  v4f acc[MB][RNB];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < RNB; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int KT = g.Np / TK, last = (KT - 1) * TK;   // KT is a multiple of 8
+  const int2 base = rfrag_base();
+  const int n_ks = KT, n_rb = g.npd / 16, rb0 = row0 / 16;
+  GSrc src;
+  AFrag<MB> a0, a1;
+  rload_g(g, wf, c0, 0, src);
+  rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 0, a0);
+  rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 1, a1);
+  rstore_g(st[0], src);
+  rload_g(g, wf, c0, TK, src);
+  __syncthreads();
+  for (int kt = 0; kt < KT; kt += 2) {
+    // stage 0: slice kt; src: slice kt + 1; a0: slice kt; a1: slice kt + 1
+    rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
+    rstore_g(st[1], src);
+    rload_g(g, wf, c0, min((kt + 2) * TK, last), src);
+    rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
+    rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
+    __syncthreads();
+    // stage 1: slice kt + 1; src: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
+    rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
+    rstore_g(st[0], src);
+    rload_g(g, wf, c0, min((kt + 3) * TK, last), src);
+    rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
+    rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
+    __syncthreads();
+  }
+
+  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
+#pragma unroll
+  for (int ti = 0; ti < MB; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < RNB; ++tj)
+      if (ti < mi && tj < nj) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = row0 + ti * 16 + 4 * lg + r;
+          const int c = cw + tj * 16 + lr;
+          Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
+        }
+      }
+}
+
+}  // namespace
+
+void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st) {
+  const size_t n8 = (size_t)rows * (cols / 8);
+  size_t grid = (n8 + 255) / 256;
+  if (grid > 65535u * 16u) grid = 65535u * 16u;
+  hipLaunchKernelGGL(k_tile_bf16x3, dim3((unsigned)grid), dim3(256), 0, st, src, rows, cols, dst);
+}
+
+void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, hipStream_t st) {
+  Gemm1RArgs g;
+  const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
+  const size_t NX = (size_t)e.Np * e.npd;
+  g.Xh = e.X_frag; g.Xl = e.X_frag + NX;
+  g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
+  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_it = e.Np / 256; g.n_ct = (e.npd + RTN - 1) / RTN;
+  g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  const int total = sb * g.n_it * g.n_ct;
+  hipLaunchKernelGGL(k_gemm1_bf16x3_rp, dim3(8 * ((total + 7) / 8)), dim3(RNT), 0, st, g);
+  launch_weights_bf16(e, sb, g.n_ct, what, st);
+}
+
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, hipStream_t st) {
+  Gemm2RArgs g;
+  const size_t NX = (size_t)e.Np * e.npd;
+  g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
+  g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B;
+  g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  g.n_mt = (e.npd + 191) / 192; g.n_ct = (e.npd + RTN - 1) / RTN;
+  const int total = sb * g.n_mt * g.n_ct;
+  hipLaunchKernelGGL(k_gemm2_bf16x3_rp, dim3(8 * ((total + 7) / 8)), dim3(RNT), 0, st, g);
+}
+
+}  // namespace tdscha

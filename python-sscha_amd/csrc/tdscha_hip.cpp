@@ -120,10 +120,11 @@ int choose_s_batch(const Ensemble &e, int requested) {
   if (per < 1) per = 1;
   long sb = (4096 + per - 1) / per;
   // workspace per symmetry: Tw, Ys, B (npd^2 doubles each) + reduced-precision Ys + W + G + V
-  // + the BF16x3 row sums
+  // + the BF16x3 row sums and FP32 weights
   const double per_s_bytes = 3.0 * 8.0 * e.npd * (double)e.npd + 4.0 * e.npd * (double)e.npd +
                              8.0 * (4.0 * e.Np + 3.0 * e.npd) +
-                             8.0 * 4.0 * e.Np * ((e.npd + kBfTileN - 1) / kBfTileN);
+                             8.0 * 4.0 * e.Np * ((e.npd + kBfTileN - 1) / kBfTileN) +
+                             4.0 * 2.0 * e.Np;
   const double budget = 12.0e9;
   long cap = (long)(budget / per_s_bytes);
   if (cap < 1) cap = 1;
@@ -240,8 +241,11 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   rc |= dev_alloc(h, &e.V, sb * e.npd);
   rc |= dev_alloc(h, &e.Vp, (size_t)((e.Np + kFavChunk - 1) / kFavChunk) * sb * e.npd);
   e.Pw = nullptr;
-  if (e.prec_mask & (1 << kBF16x3))
+  e.Wf = nullptr;
+  if (e.prec_mask & (1 << kBF16x3)) {
     rc |= dev_alloc(h, &e.Pw, sb * bf16_col_tiles(e) * 4 * (size_t)e.Np);
+    rc |= dev_alloc(h, &e.Wf, sb * 2 * (size_t)e.Np);
+  }
   e.Ys_f32 = nullptr;
   e.Ys_bf = nullptr;
   if (e.prec_mask & (1 << kFP32)) rc |= dev_alloc(h, &e.Ys_f32, sb * nn);
@@ -443,6 +447,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   if (wantbf) {
     rc |= dev_alloc(h, &e.X_hi, 2 * NX);
     rc |= dev_alloc(h, &e.XtT_hi, 2 * NX);
+    rc |= dev_alloc(h, &e.X_frag, 2 * NX);
+    rc |= dev_alloc(h, &e.XtT_frag, 2 * NX);
     e.X_lo = e.X_hi + NX;
     e.XtT_lo = e.XtT_hi + NX;
   }
@@ -488,6 +494,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   if (wantbf) {
     launch_convert(e.X, e.X_hi, NX, kBF16x3, nullptr);
     launch_convert(e.XtT, e.XtT_hi, NX, kBF16x3, nullptr);
+    launch_tile_bf16x3(e.X, Np, npd, e.X_frag, nullptr);
+    launch_tile_bf16x3(e.XtT, npd, Np, e.XtT_frag, nullptr);
   }
   hipError_t err = hipDeviceSynchronize();
   if (err == hipSuccess) err = hipGetLastError();
@@ -605,7 +613,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   const int what = kTermF | kTermD2vR | kTermD2vY;
   int best = e.s_batch, best_v = 0;
   double best_t = 1e300;
-  const int n_variants = precision == kBF16x3 ? 3 : 1;
+  const int n_variants = precision == kBF16x3 ? 4 : 1;
   for (int v = 0; v < n_variants && !rc; ++v) {
     e.bf_variant = v;
     for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
@@ -643,7 +651,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 2 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 3 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

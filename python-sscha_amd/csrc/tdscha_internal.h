@@ -85,6 +85,9 @@ struct Ensemble {
   int2 *dpairs = nullptr;
   // reduced-precision operand copies (bf16 stored as raw 16-bit words; lo = hi + count)
   uint16_t *X_hi = nullptr, *X_lo = nullptr, *XtT_hi = nullptr, *XtT_lo = nullptr;
+  // the same bf16 hi / lo planes of X and XtT in the MFMA fragment-tiled layout (hi plane,
+  // then lo plane) read by the register-operand contraction kernels (tile form 3)
+  uint16_t *X_frag = nullptr, *XtT_frag = nullptr;
   float *X_f = nullptr, *Ft_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
 
   // per-chain inputs / accumulators, for up to n_chains chains (grown on demand)
@@ -105,6 +108,7 @@ struct Ensemble {
   }
   double *G = nullptr;        // [s_batch x 2 x npd]  g1 = f_ups*S^T R1, g2 = S^T R1 / f_psi
   double *W = nullptr;        // [s_batch x 4 x Np]  c1, c2, e1, e2 per image
+  float *Wf = nullptr;        // [s_batch x 2 x Np]  c1, c2 / 2 in FP32 (BF16x3 GEMM2 operand)
   double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
   double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
@@ -114,7 +118,8 @@ struct Ensemble {
   int adj_chains = 0, adj_slots = 0;
   int pass_slots = 0;         // slots used per pass (autotuned; 0 = s_batch)
   int bf_variant = 0;         // BF16x3 contraction kernels (autotuned): 0 = 256/192 x 128,
-                              // 1 = 256 x 256, 2 = 256 x 256 with direct global->LDS copies
+                              // 1 = 256 x 256, 2 = 256 x 256 with direct global->LDS copies,
+                              // 3 = 256/192 x 192 with wave-private A operands
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
@@ -171,6 +176,13 @@ void launch_grad_out(const Ensemble &e, double *grad, int accumulate, hipStream_
 int bf16_col_tiles(const Ensemble &e);
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st);
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st);
+// per-image weights W (and their FP32 copy Wf) from GEMM1's row sums over n_ct column tiles
+void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStream_t st);
+// wave-private-operand forms (gemm_bf16_rp.hip); GEMM2 writes B_s^T
+void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, hipStream_t st);
+// fp64 [rows x cols] -> fragment-tiled bf16 hi / lo planes (dst, dst + rows * cols)
+void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st);
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, hipStream_t st);
 // V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 // GEMM1: per image (config i, symmetry s) the quadratic forms

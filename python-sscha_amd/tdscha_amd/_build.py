@@ -14,7 +14,7 @@ INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libtdscha_hip.so")
 
-SOURCES = ["tdscha_hip.cpp", "rotate.hip", "gemm.hip", "gemm_bf16.hip", "operator.hip",
+SOURCES = ["tdscha_hip.cpp", "rotate.hip", "gemm.hip", "gemm_bf16.hip", "gemm_bf16_rp.hip", "operator.hip",
            "lanczos.hip"]
 ARCH = os.environ.get("TDSCHA_OFFLOAD_ARCH", "gfx950")
 # Packed-FP32 instructions are disabled for every device pass.
