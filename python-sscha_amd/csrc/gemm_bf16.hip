@@ -669,8 +669,8 @@ void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStrea
 }
 
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
-  if (e.bf_variant == 3)
-    launch_gemm1_bf16x3_rp(e, sb, what, st);
+  if (e.bf_variant >= 3)
+    launch_gemm1_bf16x3_rp(e, sb, what, e.bf_variant == 3 ? 2 : 1, st);   // 4, 5: 96 columns
   else if (e.bf_variant == 2)
     gemm1_launch_dma(e, sb, what, st);
   else if (e.bf_variant == 1)
@@ -680,8 +680,8 @@ void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
 }
 
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
-  if (e.bf_variant == 3)
-    launch_gemm2_bf16x3_rp(e, sb, st);
+  if (e.bf_variant >= 3)
+    launch_gemm2_bf16x3_rp(e, sb, e.bf_variant == 4 ? 1 : 2, st);   // 3, 5: 192 columns
   else if (e.bf_variant == 2)
     gemm2_launch_dma(e, sb, st);
   else if (e.bf_variant == 1)

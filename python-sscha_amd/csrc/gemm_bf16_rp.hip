@@ -1,12 +1,13 @@
 // BF16x3 contractions with register-resident A operands (gfx950): tile form 3 of the
 // autotuner.
 //
-// A workgroup tile is TM x 192 with 512 threads = 8 waves, 4 row waves x 2 column waves.
-// Wave (wr, wc) owns rows [wr WR, (wr + 1) WR) and columns [96 wc, 96 wc + 96): WR / 16 x 6
-// blocks of 16 x 16, FP32 accumulators in VGPRs (two waves per SIMD). A fragments go from
+// A workgroup tile is TM x 96 WCN with 4 row waves x WCN column waves (WCN = 2: 512 threads,
+// one workgroup per CU, tile form 3; WCN = 1: 256 threads, two independent workgroups per CU,
+// tile form 4). Wave (wr, wc) owns rows [wr WR, (wr + 1) WR) and columns [96 wc, 96 wc + 96):
+// WR / 16 x 6 blocks of 16 x 16, FP32 accumulators in VGPRs (two waves per SIMD). A fragments go from
 // global memory straight into registers in the MFMA operand layout and never touch LDS (the
-// two column waves of a row load the same rows; the second load hits L2). Only the B slice
-// (192 rows x 32 k, hi and lo planes, 24 KB) is shared through LDS: two stages of unpadded
+// column waves of a row load the same rows; the second load hits L2). Only the B slice
+// (96 WCN rows x 32 k, hi and lo planes) is shared through LDS: two stages of unpadded
 // 64-byte rows, 16-byte chunk j of row r stored at chunk j ^ sw(r),
 // sw(r) = ((r >> 1) ^ (r >> 3)) & 3, so that the fragment reads (16 rows, one chunk each,
 // per 16 lanes) and the staging writes (4 rows x 4 chunks, or 2 rows x 8 half chunks, per 16
@@ -35,46 +36,57 @@ using namespace bfx;
 
 namespace {
 
-constexpr int RTN = 192;          // tile columns
 constexpr int RWC = 96;           // columns per wave
 constexpr int RNB = RWC / 16;     // 16-column blocks per wave
-constexpr int RNT = 512;          // threads: 4 row waves x 2 column waves
+
+// WCN column waves: tile columns, threads (4 row waves x WCN column waves), threads per
+// staged plane and rows per staging pass (three passes cover the tile's rows).
+template <int WCN> struct RCfg {
+  static constexpr int TN = RWC * WCN;
+  static constexpr int NT = 256 * WCN;
+  static constexpr int HALF = NT / 2;
+  static constexpr int RPP = NT / 8;
+};
 
 // Native vector registers (HIP's uint4 / float4 structs are copied with memcpy, which keeps
 // them in scratch memory).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-struct RStage {
-  uint16_t bh[RTN * TK], bl[RTN * TK];   // 2 x 12 KB
+template <int TN> struct RStage {
+  uint16_t bh[TN * TK], bl[TN * TK];     // 2 x 6 KB per 96 columns
 };
 
 __device__ inline int rswz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
 
-// ---- B slice from bf16 planes: waves 0-3 stage the hi plane, waves 4-7 the lo plane, each
-// thread rows (t % 256) / 4 + 64 h (h < 3), 16-byte chunk t % 4.
+// ---- B slice from bf16 planes: the first half of the threads stage the hi plane, the second
+// half the lo plane, each thread rows (t % HALF) / 4 + RPP h (h < 3), 16-byte chunk t % 4.
 struct BSlice {
   u32x4 c[3];
 };
 
+template <int WCN>
 __device__ inline void rload_b(const uint16_t *__restrict__ Ph, const uint16_t *__restrict__ Pl,
                                size_t ld, int rows, int r0, int k0, BSlice &b) {
-  const int t = threadIdx.x, tt = t & 255;
-  const uint16_t *P = t < 256 ? Ph : Pl;
+  using C = RCfg<WCN>;
+  const int t = threadIdx.x, tt = t % C::HALF;
+  const uint16_t *P = t < C::HALF ? Ph : Pl;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
-    int r = r0 + (tt >> 2) + 64 * h;
+    int r = r0 + (tt >> 2) + C::RPP * h;
     r = r < rows ? r : rows - 1;   // rows past the edge feed columns that are never used
     b.c[h] = *reinterpret_cast<const u32x4 *>(P + (size_t)r * ld + k0 + (tt & 3) * 8);
   }
 }
 
-__device__ inline void rstore_b(RStage &s, const BSlice &b) {
-  const int t = threadIdx.x, tt = t & 255;
-  uint16_t *dst = t < 256 ? s.bh : s.bl;
+template <int WCN>
+__device__ inline void rstore_b(RStage<RCfg<WCN>::TN> &s, const BSlice &b) {
+  using C = RCfg<WCN>;
+  const int t = threadIdx.x, tt = t % C::HALF;
+  uint16_t *dst = t < C::HALF ? s.bh : s.bl;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
-    const int row = (tt >> 2) + 64 * h;
+    const int row = (tt >> 2) + C::RPP * h;
     *reinterpret_cast<u32x4 *>(dst + row * TK + (((tt & 3) ^ rswz(row)) * 8)) = b.c[h];
   }
 }
@@ -132,8 +144,8 @@ struct BFrag {
 // B fragment of column block TJ of this wave: row 96 wc + 16 TJ + lr, k group lg, stored in
 // chunk lg ^ sw(lr) ^ 2 (TJ & 1) of that row (96 wc is a multiple of 32). `base` holds the
 // offsets of the two parities for this lane.
-template <int TJ>
-__device__ inline BFrag rfrag(const RStage &s, int2 base) {
+template <int TJ, int TN>
+__device__ inline BFrag rfrag(const RStage<TN> &s, int2 base) {
   const int o = TJ * 16 * TK + ((TJ & 1) ? base.y : base.x);
   return {__builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(s.bh + o)),
           __builtin_bit_cast(v8bf, *reinterpret_cast<const u32x4 *>(s.bl + o))};
@@ -162,8 +174,8 @@ __device__ inline void rmma_block(const AFrag<MB> &a, const BFrag &b, v4f (&acc)
 // the MFMAs of block TJ are issued (the group barriers pin that order: left alone, the
 // scheduler sinks the reads below most of the MFMAs that should cover their latency), and a
 // scheduling fence after each block keeps further reads from being hoisted.
-template <int MB, int TJ, int J1>
-__device__ inline void rmma_from(const RStage &s, const AFrag<MB> &a, const BFrag &cur,
+template <int MB, int TJ, int J1, int TN>
+__device__ inline void rmma_from(const RStage<TN> &s, const AFrag<MB> &a, const BFrag &cur,
                                  int2 base, v4f (&acc)[MB][RNB]) {
   if constexpr (TJ + 1 < J1) {
     const BFrag nxt = rfrag<TJ + 1>(s, base);
@@ -171,19 +183,19 @@ __device__ inline void rmma_from(const RStage &s, const AFrag<MB> &a, const BFra
     __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);        // the two DS reads
     __builtin_amdgcn_sched_group_barrier(0x008, 3 * MB, 0);   // then the MFMAs
     __builtin_amdgcn_sched_barrier(0);
-    rmma_from<MB, TJ + 1, J1>(s, a, nxt, base, acc);
+    rmma_from<MB, TJ + 1, J1, TN>(s, a, nxt, base, acc);
   } else {
     rmma_block<MB, TJ>(a, cur, acc);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
 
-template <int MB, int J0, int J1>
-__device__ inline void rmma(const RStage &s, const AFrag<MB> &a, int2 base,
+template <int MB, int J0, int J1, int TN>
+__device__ inline void rmma(const RStage<TN> &s, const AFrag<MB> &a, int2 base,
                             v4f (&acc)[MB][RNB]) {
   const BFrag first = rfrag<J0>(s, base);
   __builtin_amdgcn_sched_barrier(0);
-  rmma_from<MB, J0, J1>(s, a, first, base, acc);
+  rmma_from<MB, J0, J1, TN>(s, a, first, base, acc);
 }
 
 // ------------------------------------------------------------------------------ GEMM1
@@ -233,9 +245,10 @@ __device__ inline void row_sums(const Gemm1RArgs &g, int sl, int ti, int row0, i
     }
 }
 
-__global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
-  constexpr int MB = 4, TM = 256;
-  __shared__ RStage st[2];
+template <int WCN>
+__global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
+  constexpr int MB = 4, TM = 256, TN = RCfg<WCN>::TN;
+  __shared__ RStage<TN> st[2];
   const int total = g.sb * g.n_it * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
@@ -243,7 +256,7 @@ __global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
   const int sl = id / (g.n_it * g.n_ct);
   const int rest = id - sl * g.n_it * g.n_ct;
   const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
-  const int i0 = it * TM, c0 = ct * RTN;
+  const int i0 = it * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int wr = w & 3, wc = w >> 2;
   const int cw = c0 + wc * RWC;                 // first column of this wave
@@ -264,24 +277,24 @@ __global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
     const int n_ks = KT, n_rb = g.Np / 16, rb0 = row0 / 16;
     BSlice b0, b1;
     AFrag<MB> a0, a1;
-    rload_b(Yh, Yl, g.npd, g.npd, c0, 0, b0);
+    rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, 0, b0);
     rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, 0, a0);
-    rload_b(Yh, Yl, g.npd, g.npd, c0, TK, b1);
+    rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, TK, b1);
     rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, 1, a1);
-    rstore_b(st[0], b0);
+    rstore_b<WCN>(st[0], b0);
     __syncthreads();
     for (int kt = 0; kt < KT; kt += 2) {
       // stage 0: slice kt; b1: slice kt + 1; a0: slice kt; a1: slice kt + 1
-      rload_b(Yh, Yl, g.npd, g.npd, c0, min((kt + 2) * TK, last), b0);
+      rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, min((kt + 2) * TK, last), b0);
       rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
-      rstore_b(st[1], b1);
+      rstore_b<WCN>(st[1], b1);
       rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
       rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
       __syncthreads();
       // stage 1: slice kt + 1; b0: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
-      rload_b(Yh, Yl, g.npd, g.npd, c0, min((kt + 3) * TK, last), b1);
+      rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, min((kt + 3) * TK, last), b1);
       rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
-      rstore_b(st[0], b0);
+      rstore_b<WCN>(st[0], b0);
       rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
       rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
       __syncthreads();
@@ -289,9 +302,24 @@ __global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
   }
 
   // epilogue: FP64 row sums over this wave's 96 columns, one 16-row block at a time (a
-  // 16-lane butterfly adds the 16 columns of a block). Column wave 1 leaves its sums in LDS
-  // (the stages are free: the loop ended on a barrier); column wave 0 adds them to its own,
-  // in that order, and writes the tile's row sums.
+  // 16-lane butterfly adds the 16 columns of a block). With two column waves, column wave 1
+  // leaves its sums in LDS (the stages are free: the loop ended on a barrier); column wave 0
+  // adds them to its own, in that order, and writes the tile's row sums.
+  double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np;
+  if constexpr (WCN == 1) {
+#pragma unroll
+    for (int ti = 0; ti < MB; ++ti) {
+      double p[4][4];
+      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
+      if (lr == 0)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            Pt[(size_t)q * g.Np + row0 + ti * 16 + 4 * lg + r] = p[r][q];
+    }
+    return;
+  }
   double(*red)[4] = reinterpret_cast<double(*)[4]>(&st[0]);   // [256][4]: 8 KB
   if (wc == 1) {
 #pragma unroll
@@ -307,7 +335,6 @@ __global__ __launch_bounds__(RNT, 1) void k_gemm1_bf16x3_rp(Gemm1RArgs g) {
   }
   __syncthreads();
   if (wc == 0) {
-    double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np;
 #pragma unroll
     for (int ti = 0; ti < MB; ++ti) {
       double p[4][4];
@@ -332,17 +359,19 @@ struct Gemm2RArgs {
   int Np, npd, sb, n_mt, n_ct;
 };
 
-// Sources of the formed B slice: rows (t / 8) + 64 h (h < 3) of the tile, k = 4 (t % 8) .. + 4.
+// Sources of the formed B slice: rows (t / 8) + RPP h (h < 3) of the tile,
+// k = 4 (t % 8) .. + 4.
 struct GSrc {
   v4f y[3], x[3], wa, wb;
 };
 
+template <int WCN>
 __device__ inline void rload_g(const Gemm2RArgs &g, const float *__restrict__ wf, int c0, int k0,
                                GSrc &s) {
   const int t = threadIdx.x, kk = k0 + (t & 7) * 4;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
-    int r = c0 + (t >> 3) + 64 * h;
+    int r = c0 + (t >> 3) + RCfg<WCN>::RPP * h;
     r = r < g.npd ? r : g.npd - 1;
     s.y[h] = *reinterpret_cast<const v4f *>(g.YTf + (size_t)r * g.Np + kk);
     s.x[h] = *reinterpret_cast<const v4f *>(g.XtTf + (size_t)r * g.Np + kk);
@@ -353,11 +382,12 @@ __device__ inline void rload_g(const Gemm2RArgs &g, const float *__restrict__ wf
 
 // G^T = c1 Y^T + c2/2 X~^T in FP32, split into hi / lo and written as 8-byte halves of the
 // swizzled 16-byte chunks.
-__device__ inline void rstore_g(RStage &st, const GSrc &s) {
+template <int WCN>
+__device__ inline void rstore_g(RStage<RCfg<WCN>::TN> &st, const GSrc &s) {
   const int t = threadIdx.x, q = t & 7;
 #pragma unroll
   for (int h = 0; h < 3; ++h) {
-    const int row = (t >> 3) + 64 * h;
+    const int row = (t >> 3) + RCfg<WCN>::RPP * h;
     uint32_t hi0, lo0, hi1, lo1;
     split2(fmaf(s.wa.x, s.y[h].x, s.wb.x * s.x[h].x), fmaf(s.wa.y, s.y[h].y, s.wb.y * s.x[h].y),
            hi0, lo0);
@@ -369,16 +399,17 @@ __device__ inline void rstore_g(RStage &st, const GSrc &s) {
   }
 }
 
-__global__ __launch_bounds__(RNT, 1) void k_gemm2_bf16x3_rp(Gemm2RArgs g) {
-  constexpr int MB = 3, TM = 192;
-  __shared__ RStage st[2];
+template <int WCN>
+__global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm2RArgs g) {
+  constexpr int MB = 3, TM = 192, TN = RCfg<WCN>::TN;
+  __shared__ RStage<TN> st[2];
   const int total = g.sb * g.n_mt * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
   const int sl = id / (g.n_mt * g.n_ct);
   const int rest = id - sl * g.n_mt * g.n_ct;
   const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
-  const int m0 = mt * TM, c0 = ct * RTN;
+  const int m0 = mt * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int wr = w & 3, wc = w >> 2;
   const int row0 = m0 + wr * 48, cw = c0 + wc * RWC;
@@ -397,24 +428,24 @@ __global__ __launch_bounds__(RNT, 1) void k_gemm2_bf16x3_rp(Gemm2RArgs g) {
   const int n_ks = KT, n_rb = g.npd / 16, rb0 = row0 / 16;
   GSrc src;
   AFrag<MB> a0, a1;
-  rload_g(g, wf, c0, 0, src);
+  rload_g<WCN>(g, wf, c0, 0, src);
   rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 0, a0);
   rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 1, a1);
-  rstore_g(st[0], src);
-  rload_g(g, wf, c0, TK, src);
+  rstore_g<WCN>(st[0], src);
+  rload_g<WCN>(g, wf, c0, TK, src);
   __syncthreads();
   for (int kt = 0; kt < KT; kt += 2) {
     // stage 0: slice kt; src: slice kt + 1; a0: slice kt; a1: slice kt + 1
     rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
-    rstore_g(st[1], src);
-    rload_g(g, wf, c0, min((kt + 2) * TK, last), src);
+    rstore_g<WCN>(st[1], src);
+    rload_g<WCN>(g, wf, c0, min((kt + 2) * TK, last), src);
     rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
     rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
     __syncthreads();
     // stage 1: slice kt + 1; src: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
     rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
-    rstore_g(st[0], src);
-    rload_g(g, wf, c0, min((kt + 3) * TK, last), src);
+    rstore_g<WCN>(st[0], src);
+    rload_g<WCN>(g, wf, c0, min((kt + 3) * TK, last), src);
     rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
     rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
     __syncthreads();
@@ -444,7 +475,7 @@ void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hi
   hipLaunchKernelGGL(k_tile_bf16x3, dim3((unsigned)grid), dim3(256), 0, st, src, rows, cols, dst);
 }
 
-void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, hipStream_t st) {
+void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st) {
   Gemm1RArgs g;
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
   const size_t NX = (size_t)e.Np * e.npd;
@@ -452,22 +483,32 @@ void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, hipStream_t st)
   g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
   g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
-  g.n_it = e.Np / 256; g.n_ct = (e.npd + RTN - 1) / RTN;
+  const int tn = RWC * wcn;
+  g.n_it = e.Np / 256; g.n_ct = (e.npd + tn - 1) / tn;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
-  hipLaunchKernelGGL(k_gemm1_bf16x3_rp, dim3(8 * ((total + 7) / 8)), dim3(RNT), 0, st, g);
+  const dim3 grid(8 * ((total + 7) / 8));
+  if (wcn == 2)
+    hipLaunchKernelGGL(k_gemm1_bf16x3_rp<2>, grid, dim3(RCfg<2>::NT), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_gemm1_bf16x3_rp<1>, grid, dim3(RCfg<1>::NT), 0, st, g);
   launch_weights_bf16(e, sb, g.n_ct, what, st);
 }
 
-void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, hipStream_t st) {
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st) {
   Gemm2RArgs g;
   const size_t NX = (size_t)e.Np * e.npd;
   g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
   g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
-  g.n_mt = (e.npd + 191) / 192; g.n_ct = (e.npd + RTN - 1) / RTN;
+  const int tn = RWC * wcn;
+  g.n_mt = (e.npd + 191) / 192; g.n_ct = (e.npd + tn - 1) / tn;
   const int total = sb * g.n_mt * g.n_ct;
-  hipLaunchKernelGGL(k_gemm2_bf16x3_rp, dim3(8 * ((total + 7) / 8)), dim3(RNT), 0, st, g);
+  const dim3 grid(8 * ((total + 7) / 8));
+  if (wcn == 2)
+    hipLaunchKernelGGL(k_gemm2_bf16x3_rp<2>, grid, dim3(RCfg<2>::NT), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_gemm2_bf16x3_rp<1>, grid, dim3(RCfg<1>::NT), 0, st, g);
 }
 
 }  // namespace tdscha

@@ -108,10 +108,10 @@ int check_blocks(int n, int n_blocks, const int *N_deg, const int *const *deg_sp
 int choose_s_batch(const Ensemble &e, int requested) {
   if (requested > 0) return requested < e.N_sym ? requested : e.N_sym;
   // workgroups per symmetry image of the least parallel contraction (GEMM2: n x n output
-  // tiles), aiming at about 4096 workgroups per launch
+  // tiles, 192 x 192 in the BF16x3 forms), aiming at about 4096 workgroups per launch
   long per;
   if (e.prec_mask & (1 << kBF16x3)) {
-    per = (long)((e.npd + 255) / 256) * ((e.npd + kBfTileN - 1) / kBfTileN);
+    per = (long)((e.npd + 191) / 192) * ((e.npd + 191) / 192);
   } else {
     const long per1 = e.Np / 128;
     const long per2 = (long)(e.npd / kPadCols) * (e.npd / kPadCols);
@@ -613,7 +613,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   const int what = kTermF | kTermD2vR | kTermD2vY;
   int best = e.s_batch, best_v = 0;
   double best_t = 1e300;
-  const int n_variants = precision == kBF16x3 ? 4 : 1;
+  const int n_variants = precision == kBF16x3 ? 6 : 1;
   for (int v = 0; v < n_variants && !rc; ++v) {
     e.bf_variant = v;
     for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
@@ -651,7 +651,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 3 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 5 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

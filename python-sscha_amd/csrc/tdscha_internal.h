@@ -28,7 +28,7 @@ constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 
 constexpr int kRotTile = 64;   // widest block / column group of the fused rotations
 constexpr int kFavChunk = 1024; // configurations per split-k chunk of the f_av product
-constexpr int kBfTileN = 128;   // column tile of the BF16x3 contractions
+constexpr int kBfTileN = 96;    // narrowest column tile of the BF16x3 GEMM1 row sums
 
 enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
 
@@ -119,7 +119,9 @@ struct Ensemble {
   int pass_slots = 0;         // slots used per pass (autotuned; 0 = s_batch)
   int bf_variant = 0;         // BF16x3 contraction kernels (autotuned): 0 = 256/192 x 128,
                               // 1 = 256 x 256, 2 = 256 x 256 with direct global->LDS copies,
-                              // 3 = 256/192 x 192 with wave-private A operands
+                              // 3 = 256/192 x 192 with wave-private A operands,
+                              // 4 = 256/192 x 96, the same with two workgroups per CU,
+                              // 5 = GEMM1 of form 4 with GEMM2 of form 3
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
@@ -179,10 +181,10 @@ void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st);
 // per-image weights W (and their FP32 copy Wf) from GEMM1's row sums over n_ct column tiles
 void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStream_t st);
 // wave-private-operand forms (gemm_bf16_rp.hip); GEMM2 writes B_s^T
-void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, hipStream_t st);
+void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st);
 // fp64 [rows x cols] -> fragment-tiled bf16 hi / lo planes (dst, dst + rows * cols)
 void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st);
-void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, hipStream_t st);
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st);
 // V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 // GEMM1: per image (config i, symmetry s) the quadratic forms

@@ -93,13 +93,13 @@ def test_group_tiled_rotation_widths(precision, max_block, seed):
 
 @pytest.mark.parametrize("n,N,seed", [(150, 300, 31), (300, 520, 32), (45, 256, 33)])
 def test_bf16x3_tile_shapes(n, N, seed):
-    """The four contraction kernel forms against the oracle, and against each other (they
+    """The six contraction kernel forms against the oracle, and against each other (they
     differ only in the order of GEMM1's FP64 row sums)."""
     ens = fixtures.random_ensemble(seed, n, N, 4, max_block=8)
     R1, Y1 = fixtures.random_perturbation(seed + 1, n)
     dev = _dev_ens(ens, precisions=("fp64", "bf16x3"))
     out = []
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4, 5):
         dev.set_tuning(v, 0)
         assert dev.tuning()[0] == v
         out.append(_check(ens, R1, Y1, 1, dev=dev, tol=2e-4, precision="bf16x3"))
