@@ -34,6 +34,34 @@ WORKLOADS = {
     "rocksalt2": (2, 2000, "uniform"),
 }
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+# contraction kernels per BF16x3 tile form (tdscha_ensemble_set_tuning), GEMM1 and GEMM2
+BF16_KERNELS = {
+    0: ("k_gemm1_bf16x3<256, 128>", "k_gemm2_bf16x3<192, 128>"),
+    1: ("k_gemm1_bf16x3<256, 256>", "k_gemm2_bf16x3<256, 256>"),
+    2: ("k_gemm1_bf16x3_dma", "k_gemm2_bf16x3_dma"),
+    3: ("k_gemm1_bf16x3_rp<2>", "k_gemm2_bf16x3_rp<2>"),
+    4: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<1>"),
+    5: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<2>"),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bf16x3_rocksalt6.md")
+
+
+def measured_traffic(kernel, workload, slots):
+    """HBM bytes per launch of `kernel` from the committed hardware-counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this benchmark, calibrated against a
+    copy kernel), for the same workload and image slots per pass; None if not measured."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    for line in lines:
+        f = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(f) >= 4 and f[0] == kernel and f[1] == workload and f[2] == str(slots):
+            try:
+                return float(f[3])
+            except ValueError:
+                return None
+    return None
 
 
 def log(*a):
@@ -87,6 +115,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shape and pass size instead of timing them")
+    ap.add_argument("--tiles", type=int, default=-1,
+                    help="BF16x3 tile form 0-5 (with --pass-slots: no autotuning)")
+    ap.add_argument("--pass-slots", type=int, default=0, help="image slots per pass")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB on the device "
                          "(a kernel of known traffic for hardware-counter calibration)")
@@ -131,7 +162,10 @@ def main():
     def step():
         dens.apply_full_L_batched(psis, flags, shifts, precision=args.precision, outs=outs)
 
-    if not args.no_autotune:
+    if args.tiles >= 0:
+        dens.set_tuning(args.tiles, args.pass_slots)
+        log("tuning set: tiles %d, %d slots per pass" % dens.tuning())
+    elif not args.no_autotune:
         t0 = time.time()
         slots, per = dens.autotune(args.precision, args.chains)
         log("autotune: tiles %d, %d slots per pass (%.3g us per image) in %.1f s" % (
@@ -176,9 +210,15 @@ def main():
     per_launch_flop = flop_l * args.chains * share[dom] / (cnt / args.steps)
     achieved = per_launch_flop / (ms / cnt * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.precision]
+    if args.precision == "bf16x3":
+        kernel = BF16_KERNELS[dens.tuning()[0]][0 if dom == "gemm1" else 1]
+    else:
+        kernel = "k_%s<%s>" % (dom, "double" if args.precision == "fp64" else "float")
+    slots_run = (dens.tuning()[1] // args.chains) * args.chains
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                "kernel": "k_%s_%s" % (dom, "f64" if args.precision == "fp64" else args.precision)}
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+                "traffic": measured_traffic(kernel, args.workload, slots_run),
+                "kernel": kernel, "slots_per_launch": slots_run}
     value = args.chains * args.steps / elapsed
     fp64_ref = None
     if args.precision != "fp64" and args.fp64_sample_syms != 0:

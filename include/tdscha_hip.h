@@ -160,9 +160,10 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
                                 const int *flags, const double *shifts,
                                 const double *const *psi, double *const *out, void *stream);
 
-/* Autotuner (no reference counterpart): times two passes of the batched L-application for
- * n_chains chains at slot counts s_batch, s_batch/2, ... (at most six) and, for BF16x3, the
- * four contraction kernel forms, and keeps the fastest per image for this ensemble. Returns
+/* Autotuner (no reference counterpart): times the batched L-application for n_chains chains
+ * (at least four passes and 168 symmetry images, best of two) at slot counts s_batch,
+ * s_batch/2, ... (at most six) and, for BF16x3, the six contraction kernel forms, and keeps
+ * the fastest per image for this ensemble. Returns
  * the chosen slot count and its time per (chain x configuration x symmetry) image in
  * microseconds (either may be NULL). Call before capturing Lanczos graphs on the ensemble. */
 int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, int *slots,
@@ -170,7 +171,8 @@ int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, 
 
 /* The tuning the autotuner sets, set or read directly: bf16_tiles 0 (256/192 x 128
  * contraction tiles), 1 (256 x 256), 2 (256 x 256, operands copied straight into
- * double-buffered LDS) or 3 (256/192 x 192, A operands in registers, only B through LDS);
+ * double-buffered LDS), 3 (256/192 x 192, A operands in registers, only B through LDS),
+ * 4 (the same with 96-column tiles, two workgroups per CU) or 5 (GEMM1 of 4, GEMM2 of 3);
  * pass_slots image slots per pass (0 = all s_batch).
  * Results do not depend on pass_slots; the tile shape only changes the order of the FP64
  * row sums of GEMM1. */

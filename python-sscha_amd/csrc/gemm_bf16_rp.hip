@@ -406,8 +406,11 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
   const int total = g.sb * g.n_mt * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
-  const int sl = id / (g.n_mt * g.n_ct);
-  const int rest = id - sl * g.n_mt * g.n_ct;
+  // slot fastest: both operands (X~^T rows and the Y^T, X~^T sources of G) are the same for
+  // every image slot, only the weights differ, so the tiles an XCD runs together share
+  // their operand panels in its L2
+  const int sl = id % g.sb;
+  const int rest = id / g.sb;
   const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
   const int m0 = mt * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
