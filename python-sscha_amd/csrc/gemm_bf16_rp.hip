@@ -24,7 +24,7 @@
 //
 // GEMM1 (WR = 64, TM = 256): Q = X Ys_s with the FP64 row sums of the tile (alpha, beta,
 //   gamma, delta) as k_gemm1_bf16x3; the two column waves of a row add theirs through LDS.
-// GEMM2 (WR = 48, TM = 192): B_s = X~^T G_s with the B operand
+// GEMM2 (WR = 48, TM = 192): B_s = X~^T G_s (stored in FP32, as accumulated) with the B operand
 //   G_s^T[c][k] = c1_k Y^T[c][k] + c2_k/2 X~^T[c][k] formed once per workgroup from the FP32
 //   operand copies while it is staged, with the arithmetic of k_gemm2_bf16x3.
 #include "tdscha_internal.h"
@@ -355,7 +355,7 @@ struct Gemm2RArgs {
   const uint16_t *Ah, *Al;    // X~^T planes [npd x Np] fragment-tiled
   const float *YTf, *XtTf;    // [npd x Np] sources of the formed operand
   const float *Wf;            // [s_batch x 2 x Np]  c1, c2 / 2
-  double *B;                  // [s_batch x npd x npd]
+  float *B;                   // [s_batch x npd x npd]
   int Np, npd, sb, n_mt, n_ct;
 };
 
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
     __syncthreads();
   }
 
-  double *Bo = g.B + (size_t)sl * g.npd * g.npd;
+  float *Bo = g.B + (size_t)sl * g.npd * g.npd;
 #pragma unroll
   for (int ti = 0; ti < MB; ++ti)
 #pragma unroll
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
         for (int r = 0; r < 4; ++r) {
           const int m = row0 + ti * 16 + 4 * lg + r;
           const int c = cw + tj * 16 + lr;
-          Bo[(size_t)m * g.npd + c] = (double)acc[ti][tj][r];
+          Bo[(size_t)m * g.npd + c] = acc[ti][tj][r];
         }
       }
 }
@@ -502,7 +502,7 @@ void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st) 
   Gemm2RArgs g;
   const size_t NX = (size_t)e.Np * e.npd;
   g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
-  g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B;
+  g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B_f;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   const int tn = RWC * wcn;
   g.n_mt = (e.npd + 191) / 192; g.n_ct = (e.npd + tn - 1) / tn;

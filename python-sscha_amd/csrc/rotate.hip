@@ -242,8 +242,8 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
 // grid (tile pairs g <= h, chains). The running sum of the tile is held in the MFMA
 // accumulators; every image of the pass is added onto it in symmetry order, so the result
 // does not depend on how the symmetries are split into passes.
-template <int G>
-__global__ __launch_bounds__(256) void k_rot2_grp(const double *__restrict__ B,
+template <int G, typename TB>
+__global__ __launch_bounds__(256) void k_rot2_grp(const TB *__restrict__ B,
                                                   double *__restrict__ d2v,
                                                   const double *__restrict__ S,
                                                   const int2 *__restrict__ groups,
@@ -270,17 +270,24 @@ __global__ __launch_bounds__(256) void k_rot2_grp(const double *__restrict__ B,
   for (int t = 0; t < ps.ns; ++t) {
     const int sl = ch * ps.ns + t;
     const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
-    const double *Bs = B + (size_t)sl * npd * npd;
+    const TB *Bs = B + (size_t)sl * npd * npd;
     __syncthreads();
+    // M = B[g][h] + (B[h][g])^T: both tiles read along their rows (coalesced), the second
+    // staged in T and added transposed
     for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
       const int r = idx / G, c = idx - r * G;
-      double v = 0.0;
-      if (r < gg.y && c < gh.y)
-        v = Bs[(size_t)(gg.x + r) * npd + gh.x + c] + Bs[(size_t)(gh.x + c) * npd + gg.x + r];
-      M[r * LD + c] = v;
+      M[r * LD + c] = (r < gg.y && c < gh.y) ? (double)Bs[(size_t)(gg.x + r) * npd + gh.x + c]
+                                             : 0.0;
+      T[r * LD + c] = (r < gh.y && c < gg.y) ? (double)Bs[(size_t)(gh.x + r) * npd + gg.x + c]
+                                             : 0.0;
     }
     build_group_s<G>(Sg, Ss, gg, meta);
     build_group_s<G>(Sh, Ss, gh, meta);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
+      const int r = idx / G, c = idx - r * G;
+      M[r * LD + c] += T[c * LD + r];
+    }
     __syncthreads();
     v4d tq[TPW];
 #pragma unroll
@@ -462,14 +469,21 @@ void launch_rot_sym(const Ensemble &e, const Pass &ps, const double *in, double 
   }
 }
 
-void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st) {
+void launch_rot2_fused(const Ensemble &e, const Pass &ps, bool b_f32, hipStream_t st) {
   dim3 grid(e.n_pairs, ps.nc);
   if (e.grp_width == 32)
-    hipLaunchKernelGGL(k_rot2_grp<32>, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dgroups,
-                       e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+    if (b_f32)
+      hipLaunchKernelGGL((k_rot2_grp<32, float>), grid, dim3(256), 0, st, e.B_f, e.d2v_acc, e.S,
+                         e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+    else
+      hipLaunchKernelGGL((k_rot2_grp<32, double>), grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S,
+                         e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+  else if (b_f32)
+    hipLaunchKernelGGL((k_rot2_grp<64, float>), grid, dim3(256), 0, st, e.B_f, e.d2v_acc, e.S,
+                       e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
   else
-    hipLaunchKernelGGL(k_rot2_grp<64>, grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S, e.dgroups,
-                       e.dpairs, e.meta, e.npd, e.sym_stride, ps);
+    hipLaunchKernelGGL((k_rot2_grp<64, double>), grid, dim3(256), 0, st, e.B, e.d2v_acc, e.S,
+                       e.dgroups, e.dpairs, e.meta, e.npd, e.sym_stride, ps);
 }
 
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {

@@ -124,7 +124,7 @@ int choose_s_batch(const Ensemble &e, int requested) {
   const double per_s_bytes = 3.0 * 8.0 * e.npd * (double)e.npd + 4.0 * e.npd * (double)e.npd +
                              8.0 * (4.0 * e.Np + 3.0 * e.npd) +
                              8.0 * 4.0 * e.Np * ((e.npd + kBfTileN - 1) / kBfTileN) +
-                             4.0 * 2.0 * e.Np;
+                             4.0 * 2.0 * e.Np + 4.0 * e.npd * (double)e.npd;
   const double budget = 12.0e9;
   long cap = (long)(budget / per_s_bytes);
   if (cap < 1) cap = 1;
@@ -207,7 +207,7 @@ int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st, in
       }
       Span sp(TDSCHA_KGROUP_ROTATE, st);
       if (e.fused_rot) {
-        launch_rot2_fused(e, ps, st);
+        launch_rot2_fused(e, ps, prec == kBF16x3 && e.bf_variant >= 3, st);
       } else {
         launch_rot_right(e, e.B, false, true, e.Tw, ps, true, st);
         launch_rot_left(e, e.Tw, e.d2v_acc, ps, false, true, st);
@@ -242,9 +242,11 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   rc |= dev_alloc(h, &e.Vp, (size_t)((e.Np + kFavChunk - 1) / kFavChunk) * sb * e.npd);
   e.Pw = nullptr;
   e.Wf = nullptr;
+  e.B_f = nullptr;
   if (e.prec_mask & (1 << kBF16x3)) {
     rc |= dev_alloc(h, &e.Pw, sb * bf16_col_tiles(e) * 4 * (size_t)e.Np);
     rc |= dev_alloc(h, &e.Wf, sb * 2 * (size_t)e.Np);
+    rc |= dev_alloc(h, &e.B_f, sb * nn);
   }
   e.Ys_f32 = nullptr;
   e.Ys_bf = nullptr;
@@ -254,7 +256,8 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   if (hipMemset(e.Ys, 0, sb * nn * 8) != hipSuccess ||
       (e.Ys_f32 && hipMemset(e.Ys_f32, 0, sb * nn * 4) != hipSuccess) ||
       (e.Ys_bf && hipMemset(e.Ys_bf, 0, sb * nn * 4) != hipSuccess) ||
-      hipMemset(e.B, 0, sb * nn * 8) != hipSuccess) {
+      hipMemset(e.B, 0, sb * nn * 8) != hipSuccess ||
+      (e.B_f && hipMemset(e.B_f, 0, sb * nn * 4) != hipSuccess)) {
     set_error("workspace initialisation failed");
     return 10;
   }

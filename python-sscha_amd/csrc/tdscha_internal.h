@@ -110,6 +110,7 @@ struct Ensemble {
   double *W = nullptr;        // [s_batch x 4 x Np]  c1, c2, e1, e2 per image
   float *Wf = nullptr;        // [s_batch x 2 x Np]  c1, c2 / 2 in FP32 (BF16x3 GEMM2 operand)
   double *B = nullptr;        // [s_batch x npd x npd]  X~^T G_s
+  float *B_f = nullptr;       // the same in FP32 (BF16x3 tile forms 3-5: FP32 accumulators)
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
   double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
   double *Pw = nullptr;       // [s_batch x col tiles x 4 x Np]  BF16x3 GEMM1 row sums
@@ -148,8 +149,9 @@ void launch_rot_sym(const Ensemble &e, const Pass &ps, const double *in, double 
                     const double *scale, hipStream_t st);
 // Ys (fp64 when prec is FP64, else the reduced-precision planes) = S^T Ysym S, fused
 void launch_rot1_fused(const Ensemble &e, const Pass &ps, int prec, hipStream_t st);
-// d2v_acc[chain] += sum over the chain's slots of S (B_s + B_s^T) S^T, fused
-void launch_rot2_fused(const Ensemble &e, const Pass &ps, hipStream_t st);
+// d2v_acc[chain] += sum over the chain's slots of S (B_s + B_s^T) S^T, fused; B_s from B_f
+// (FP32) when b_f32, else from B
+void launch_rot2_fused(const Ensemble &e, const Pass &ps, bool b_f32, hipStream_t st);
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st);
 void launch_zero(double *p, size_t count, hipStream_t st);
 void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f_out, double *d2v_out,
