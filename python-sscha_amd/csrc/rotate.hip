@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
                                                   const ModeMeta *__restrict__ meta,
                                                   const double *__restrict__ scale, int npd,
                                                   int sym_stride, Pass ps) {
-  constexpr int LD = GrpTile<G>::LD, TPW = GrpTile<G>::TPW, NT = G / 16;
+  constexpr int LD = GrpTile<G>::LD, TPW = GrpTile<G>::TPW;
   __shared__ double M[G * LD], T[G * LD], Sg[G * LD], Sh[G * LD];
   const int2 pr = pairs[blockIdx.x];
   const int2 gg = groups[pr.x], gh = groups[pr.y];
@@ -204,7 +204,6 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
     const int r = idx / G, c = idx - r * G;
     M[r * LD + c] = (r < gg.y && c < gh.y) ? Yc[(size_t)(gg.x + r) * npd + gh.x + c] : 0.0;
   }
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   for (int t = blockIdx.y; t < ps.ns; t += gridDim.y) {
     const int sl = ch * ps.ns + t;
     const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
@@ -221,19 +220,23 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
 #pragma unroll
     for (int q = 0; q < TPW; ++q) acc[q] = (v4d){0.0, 0.0, 0.0, 0.0};
     grp_mma<G, true, false>(Sg, T, acc);            // Ys = S_g^T T
+    // the tile goes out through LDS (Sh is free: its last reads were before the barrier
+    // above), so that both it and its mirror are written along rows by consecutive threads
+    grp_store_lds<G>(Sh, acc);
+    __syncthreads();
     const size_t base = (size_t)sl * npd * npd;
-#pragma unroll
-    for (int q = 0; q < TPW; ++q) {
-      const int tt = w + 4 * q, ti = tt / NT, tj = tt - ti * NT;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = ti * 16 + lg + 4 * r, j = tj * 16 + lr;
-        if (i < gg.y && j < gh.y) {
-          const double v = scale ? acc[q][r] * scale[gg.x + i] * scale[gh.x + j] : acc[q][r];
-          store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gg.x + i) * npd + gh.x + j, v);
-          if (pr.x != pr.y)
-            store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gh.x + j) * npd + gg.x + i, v);
-        }
+    for (int idx = threadIdx.x; idx < gg.y * gh.y; idx += blockDim.x) {
+      const int i = idx / gh.y, j = idx - i * gh.y;
+      const double v = Sh[i * LD + j];
+      store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gg.x + i) * npd + gh.x + j,
+               scale ? v * scale[gg.x + i] * scale[gh.x + j] : v);
+    }
+    if (pr.x != pr.y) {
+      for (int idx = threadIdx.x; idx < gg.y * gh.y; idx += blockDim.x) {
+        const int j = idx / gg.y, i = idx - j * gg.y;   // mirror row gh.x + j, column gg.x + i
+        const double v = Sh[i * LD + j];
+        store_ys(Ys, lp, lp_prec, lp_plane, base + (size_t)(gh.x + j) * npd + gg.x + i,
+                 scale ? v * scale[gg.x + i] * scale[gh.x + j] : v);
       }
     }
   }
