@@ -41,6 +41,27 @@ __device__ inline int clamp_blocks(int valid, int start, int cap) {
   return __builtin_amdgcn_readfirstlane(b);
 }
 
+// Sum of a double over each 16-lane row of the wave, in every lane of the row, with DPP
+// moves (VALU, no LDS): pairs within quads (quad_perm [1,0,3,2], [2,3,0,1]), then the
+// mirrored quad of the half row (row_half_mirror) and the mirrored half (row_mirror). Every
+// lane adds the same two partial sums at each step, so all lanes hold the same value.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+  const int2 u = __builtin_bit_cast(int2, v);
+  int2 r;
+  r.x = __builtin_amdgcn_mov_dpp(u.x, CTRL, 0xF, 0xF, false);
+  r.y = __builtin_amdgcn_mov_dpp(u.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, r);
+}
+
+__device__ inline double row16_sum(double v) {
+  v += dpp_f64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141>(v);   // row_half_mirror
+  v += dpp_f64<0x140>(v);   // row_mirror
+  return v;
+}
+
 __device__ inline v4f mfma_bf16(v8bf a, v8bf b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }

@@ -209,7 +209,7 @@ struct Gemm1RArgs {
 };
 
 // FP64 row sums of block row ti of this wave over its columns below nj (alpha, beta, gamma,
-// delta; the first two from Q), reduced over the 16 lanes of each row group.
+// delta; the first two from Q), reduced over the 16 lanes of each row group with DPP.
 template <int MB>
 __device__ inline void row_sums(const Gemm1RArgs &g, int sl, int ti, int row0, int cw, int nj,
                                 const v4f (&acc)[MB][RNB], double (&p)[4][4]) {
@@ -237,12 +237,7 @@ __device__ inline void row_sums(const Gemm1RArgs &g, int sl, int ti, int row0, i
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double v = p[r][q];
-#pragma unroll
-      for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-      p[r][q] = v;
-    }
+    for (int q = 0; q < 4; ++q) p[r][q] = row16_sum(p[r][q]);
 }
 
 template <int WCN>
