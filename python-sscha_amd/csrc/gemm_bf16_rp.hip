@@ -297,9 +297,9 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
   }
 
   // epilogue: FP64 row sums over this wave's 96 columns, one 16-row block at a time (a
-  // 16-lane butterfly adds the 16 columns of a block). With two column waves, column wave 1
-  // leaves its sums in LDS (the stages are free: the loop ended on a barrier); column wave 0
-  // adds them to its own, in that order, and writes the tile's row sums.
+  // 16-lane DPP reduction adds the 16 columns of a block). With two column waves, both leave
+  // their sums in LDS (the stages are free: the loop ended on a barrier) and the workgroup
+  // adds the two halves, wave 0's first, in coalesced stores of the tile's row sums.
   double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np;
   if constexpr (WCN == 1) {
 #pragma unroll
@@ -313,10 +313,9 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
           for (int q = 0; q < 4; ++q)
             Pt[(size_t)q * g.Np + row0 + ti * 16 + 4 * lg + r] = p[r][q];
     }
-    return;
-  }
-  double(*red)[4] = reinterpret_cast<double(*)[4]>(&st[0]);   // [256][4]: 8 KB
-  if (wc == 1) {
+  } else {
+    double(*red)[TM][4] = reinterpret_cast<double(*)[TM][4]>(&st[0]);   // [2][256][4]: 16 KB
+    static_assert(sizeof(double) * 2 * TM * 4 <= sizeof(st), "row-sum halves exceed the stages");
 #pragma unroll
     for (int ti = 0; ti < MB; ++ti) {
       double p[4][4];
@@ -325,22 +324,12 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) red[wr * 64 + ti * 16 + 4 * lg + r][q] = p[r][q];
+          for (int q = 0; q < 4; ++q) red[wc][wr * 64 + ti * 16 + 4 * lg + r][q] = p[r][q];
     }
-  }
-  __syncthreads();
-  if (wc == 0) {
-#pragma unroll
-    for (int ti = 0; ti < MB; ++ti) {
-      double p[4][4];
-      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
-      if (lr == 0)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = wr * 64 + ti * 16 + 4 * lg + r;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) Pt[(size_t)q * g.Np + i0 + m] = p[r][q] + red[m][q];
-        }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * TM; t += RCfg<WCN>::NT) {
+      const int q = t / TM, m = t - q * TM;
+      Pt[(size_t)q * g.Np + i0 + m] = red[0][m][q] + red[1][m][q];
     }
   }
 }
