@@ -230,6 +230,23 @@ int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *cons
 int tdscha_lanczos_group_run(tdscha_lanczos_group *g, int n_steps, void *stream);
 int tdscha_lanczos_group_destroy(tdscha_lanczos_group *g);
 
+/* ---- Ensemble projection on the modes ------------------------------------------------
+ * Replaces the projection inside Lanczos.__init__ (Modules/DynamicalLanczos.py:380-442,
+ * X = u_tilde . pols and Y = f_tilde . pols at Py:438-439), fused with the unit and mass
+ * rescaling of Py:385-426:
+ *   X = (u / bohr * sqrt(m)) . pols
+ *   Y = ((forces - sscha_forces) * bohr - f_mean * bohr) / sqrt(m) . pols
+ * Device pointers, FP64, row-major: u, forces, sscha_forces [n_configs x n_cart]
+ * (Angstrom, Ry/Angstrom); f_mean [n_cart] (the symmetrized average force tiled over the
+ * supercell, Ry/Angstrom); masses [n_cart] (one per Cartesian component, Ry units);
+ * pols [n_cart x n_modes] (kept modes only); X, Y [n_configs x n_modes]. sscha_forces and
+ * f_mean may be NULL (zero). bohr: Angstrom per Bohr. Enqueued on `stream` (NULL = the
+ * null stream); returns once launched. */
+int tdscha_project_ensemble(int n_configs, int n_cart, int n_modes, const double *u,
+                            const double *forces, const double *sscha_forces,
+                            const double *f_mean, const double *masses, const double *pols,
+                            double bohr, double *X, double *Y, void *stream);
+
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;
  * query synchronises on them and returns the summed elapsed time and the launch count. */

@@ -22,6 +22,7 @@ Contents
   - Wigner dot-product mask                 Py:5790-5839
   - biconjugate / symmetric Lanczos run_FT  Py:5843-6372
   - continued-fraction Green function       Py:4951-5047
+  - ensemble projection on the modes        Py:380-442 (``project_ensemble``)
 * The derivative of a linear functional of the kernel outputs with respect to the
   configuration forces Y (``force_grad``), image by image in the reference's own form
   (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
@@ -663,3 +664,27 @@ def green_function_continued_fraction(a, b, c, w_array, use_terminator=True, las
     if not use_wigner:
         return gf * perturbation_modulus
     return (-np.real(gf) + 1j * np.imag(gf)) * perturbation_modulus
+
+
+def project_ensemble(u_disps, forces, sscha_forces, f_mean, masses, pols, bohr):
+    """X, Y of Lanczos.__init__, step by step as Py:384-439 does them (FP64 numpy).
+
+    u_disps, forces, sscha_forces: (N, 3 nat) Angstrom / Ry/Angstrom; f_mean: (3 nat,) the
+    symmetrized average force tiled over the supercell (Py:391-400); masses: (3 nat,);
+    pols: (3 nat, n) kept modes. sscha_forces / f_mean may be None (zero).
+    """
+    N, nc = np.shape(u_disps)
+    u = np.array(u_disps, dtype=np.float64) / bohr                      # Py:385
+    f = np.array(forces, dtype=np.float64).reshape(N, nc).copy()        # Py:387
+    if sscha_forces is not None:
+        f -= np.asarray(sscha_forces, dtype=np.float64).reshape(N, nc)  # Py:388
+    fm = np.zeros(nc) if f_mean is None else np.array(f_mean, dtype=np.float64).ravel()
+    fm = fm * bohr                                                      # Py:400
+    f *= bohr                                                           # Py:411
+    f[:, :] -= np.tile(fm, (N, 1))                                      # Py:422
+    m = np.asarray(masses, dtype=np.float64)
+    u *= np.tile(np.sqrt(m), (N, 1))                                    # Py:425
+    f /= np.tile(np.sqrt(m), (N, 1))                                    # Py:426
+    X = u.dot(pols)                                                     # Py:438
+    Y = f.dot(pols)                                                     # Py:439
+    return X, Y

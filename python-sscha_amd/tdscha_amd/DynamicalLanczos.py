@@ -14,9 +14,11 @@ psi <-> (R1, Y1) transforms, the recurrence scalars a, b, c, the optional Gram-S
 and the continued fraction.
 
 Deviations from the reference, on purpose:
-  * The ensemble is given already projected on the modes (X, Y, w, rho, T and the
+  * The ensemble is given either already projected on the modes (X, Y, w, rho, T and the
     symmetry blocks), as Lanczos.__init__ leaves it (Py:380-442) and save_status stores it
-    (Py:3617-3646); CellConstructor / sscha objects are not required.
+    (Py:3617-3646), or as raw configurations with the dynamical matrix's w and pols
+    (set_ensemble_from_configs, projected on the device); CellConstructor / sscha objects
+    are not required.
   * Errors raise ValueError / RuntimeError instead of exiting the interpreter.
   * run_FT(full_gs=True) adds a two-sided Gram-Schmidt against every stored vector with
     the biorthogonal duals p_j * s_norm_j / c_(j-1); n_rep_orth keeps the reference's
@@ -98,7 +100,13 @@ class Lanczos(object):
         self.ignore_small_w = False
         self.L_linop = None
         self._dev_ens = None
-        if ensemble is not None:
+        if ensemble is not None and hasattr(ensemble, "u_disps"):
+            self.set_ensemble_from_configs(
+                ensemble.u_disps, ensemble.forces, ensemble.rho, ensemble.T, ensemble.m,
+                ensemble.w, ensemble.pols, getattr(ensemble, "sscha_forces", None),
+                getattr(ensemble, "f_mean", None), ignore_small_w=getattr(
+                    ensemble, "ignore_small_w", False))
+        elif ensemble is not None:
             self.set_ensemble(ensemble.X, ensemble.Y, ensemble.w, ensemble.rho, ensemble.T,
                               getattr(ensemble, "pols", None), getattr(ensemble, "m", None))
             sym = getattr(ensemble, "sym", None)
@@ -125,6 +133,34 @@ class Lanczos(object):
         self.psi = np.zeros(n + n * (n + 1), dtype=TYPE_DP)
         self._dev_ens = None
         self.initialized = False
+
+    def set_ensemble_from_configs(self, u_disps, forces, rho, T, masses, w, pols,
+                                  sscha_forces=None, f_mean=None, select_modes=None,
+                                  ignore_small_w=False, eps_w=1e-8, bohr=None):
+        """Project raw configurations on the SCHA modes on the device (Py:330-442).
+
+        u_disps, forces, sscha_forces: (N, nat, 3) or (N, 3 nat), Angstrom / Ry/Angstrom.
+        masses: (nat,) or (3 nat,); w, pols: frequencies [Ry] and mass-scaled polarization
+        vectors (3 nat, 3 nat) of the supercell dynamical matrix, translations included (they
+        are removed here, Py:333-359). f_mean: symmetrized average force over the supercell,
+        Ry/Angstrom (the reference symmetrizes it with the crystal's space group, Py:391-397;
+        pass None when it vanishes by symmetry).
+        """
+        import torch
+        from . import projection as pj
+
+        pols = np.asarray(pols, dtype=TYPE_DP)
+        good = pj.good_modes(w, pols, masses, ignore_small_w, select_modes, eps_w)
+        self.ignore_small_w = bool(ignore_small_w)
+        w_sel = np.asarray(w, dtype=TYPE_DP)[good]
+        pols_sel = pols[:, good]
+        if len(w_sel) == 1:
+            pols_sel = pols_sel.reshape((pols.shape[0], 1))
+        m = pj._masses_per_component(masses, pols.shape[0])
+        X, Y = pj.project_ensemble(u_disps, forces, m, pols_sel, sscha_forces, f_mean,
+                                   pj.BOHR if bohr is None else bohr,
+                                   device=torch.device("cuda", self.device))
+        self.set_ensemble(X.cpu().numpy(), Y.cpu().numpy(), w_sel, rho, T, pols_sel, m)
 
     def set_symmetries(self, symmetries, N_degeneracy, degenerate_space, block_id):
         """Mode-basis symmetry blocks (what prepare_symmetrization stores, Py:741-778)."""
@@ -519,6 +555,115 @@ class Lanczos(object):
             ignore_small_w=self.ignore_small_w, sym_julia=sym_julia, deg_julia=deg_julia,
             n_syms=self.n_syms, N_degeneracy=self.N_degeneracy
             if self.N_degeneracy is not None else np.zeros(0, dtype=np.intc))
+
+    def prepare_input_files(self, root_name="tdscha", n_steps=100, start_from_scratch=True,
+                            directory=".", run_symm=False):
+        """Text inputs of the reference's standalone executable (Py:862-919), same files and
+        formats: XXX.json, .ndegs, .blockid, .masses, .freqs, .rho, .X.dat, .Y.dat, .psi,
+        .block<i>, .symsb<i>."""
+        import json
+        if self.symmetries is None:
+            raise ValueError("symmetries are not set")
+        n_syms = self.symmetries[0].shape[0]
+        json_data = {"T": self.T, "n_steps": n_steps, "ignore_v2": self.ignore_harmonic,
+                     "ignore_v3": self.ignore_v3, "ignore_v4": self.ignore_v4,
+                     "use_wigner": self.use_wigner, "run_sym": run_symm,
+                     "data": {"n_configs": int(self.N), "n_modes": int(self.n_modes),
+                              "n_syms": int(n_syms), "n_blocks": len(self.symmetries),
+                              "perturbation_modulus": self.perturbation_modulus,
+                              "reverse": self.reverse_L, "shift": self.shift_value}}
+        if not start_from_scratch:
+            raise NotImplementedError("Error, restarting from a previous calculation is not "
+                                      "yet implemented.")
+        root = os.path.join(directory, root_name)
+        os.makedirs(directory, exist_ok=True)
+        with open(root + ".json", "w") as fp:
+            json.dump(json_data, fp)
+        np.savetxt(root + ".ndegs", self.N_degeneracy, fmt="%d")
+        np.savetxt(root + ".blockid", self.sym_block_id, fmt="%d")
+        np.savetxt(root + ".masses", self.m)
+        np.savetxt(root + ".freqs", self.w)
+        np.savetxt(root + ".rho", self.rho)
+        np.savetxt(root + ".X.dat", self.X)
+        np.savetxt(root + ".Y.dat", self.Y)
+        np.savetxt(root + ".psi", self.psi)
+        for i, S in enumerate(self.symmetries):
+            np.savetxt(root + ".block{:d}".format(i), self.degenerate_space[i], fmt="%d")
+            ns, b1, b2 = S.shape
+            with open(root + ".symsb{:d}".format(i), "w") as fp:
+                for isym in range(ns):
+                    for k1 in range(b1):
+                        fp.write("".join(" {:22.16f}".format(v) for v in S[isym, k1]))
+                        fp.write("\n")
+                    fp.write("\n")
+
+    def read_input_files(self, root_name="tdscha", directory="."):
+        """Load what prepare_input_files wrote (the inputs the standalone executable reads):
+        ensemble, symmetry blocks, psi and the json flags."""
+        import json
+        root = os.path.join(directory, root_name)
+        with open(root + ".json") as fp:
+            meta = json.load(fp)
+        data = meta["data"]
+        n, N = int(data["n_modes"]), int(data["n_configs"])
+        X = np.loadtxt(root + ".X.dat", ndmin=2).reshape(N, n)
+        Y = np.loadtxt(root + ".Y.dat", ndmin=2).reshape(N, n)
+        w = np.loadtxt(root + ".freqs", ndmin=1)
+        rho = np.loadtxt(root + ".rho", ndmin=1)
+        m = np.loadtxt(root + ".masses", ndmin=1) if os.path.exists(root + ".masses") else None
+        self.set_ensemble(X, Y, w, rho, float(meta["T"]), None, m)
+        ndegs = np.loadtxt(root + ".ndegs", dtype=np.intc, ndmin=1)
+        bid = np.loadtxt(root + ".blockid", dtype=np.intc, ndmin=1)
+        ns = int(data["n_syms"])
+        syms, degs = [], []
+        for i in range(int(data["n_blocks"])):
+            d = int(ndegs[i])
+            degs.append(np.loadtxt(root + ".block{:d}".format(i), dtype=np.intc, ndmin=1))
+            syms.append(np.loadtxt(root + ".symsb{:d}".format(i), ndmin=1).reshape(ns, d, d))
+        self.set_symmetries(syms, ndegs, degs, bid)
+        self.psi = np.loadtxt(root + ".psi", ndmin=1)
+        self.ignore_harmonic = bool(meta["ignore_v2"])
+        self.ignore_v3 = bool(meta["ignore_v3"])
+        self.ignore_v4 = bool(meta["ignore_v4"])
+        self.use_wigner = bool(meta["use_wigner"])
+        self.perturbation_modulus = data["perturbation_modulus"]
+        self.reverse_L = bool(data["reverse"])
+        self.shift_value = data["shift"]
+        return meta
+
+    def load_from_input_files(self, root_name="tdscha", directory="."):
+        """Results of the standalone executable (Py:920-998): XXX.abc, and the Krylov bases
+        and s norms when present (XXX.qbasis.out, .pbasis.out, .snorm.out), then the json."""
+        import json
+        import warnings
+        if not os.path.exists(directory):
+            raise IOError("Error, the specified directory '{}' does not exist".format(directory))
+        abc_file = os.path.join(directory, "{}.abc".format(root_name))
+        if not os.path.exists(abc_file):
+            raise IOError("Error, the file '{}' does not exist.".format(abc_file))
+        abc = np.loadtxt(abc_file, dtype=np.double, ndmin=2)
+        self.a_coeffs = abc[:, 0]
+        self.b_coeffs = abc[:, 1]
+        self.c_coeffs = abc[:, 2]
+        for attr, ext in (("basis_Q", "qbasis.out"), ("basis_P", "pbasis.out"),
+                          ("s_norm", "snorm.out")):
+            fn = os.path.join(directory, "{}.{}".format(root_name, ext))
+            if os.path.exists(fn):
+                v = np.loadtxt(fn, dtype=np.double)
+                setattr(self, attr, v.ravel() if attr == "s_norm" else v)
+            else:
+                warnings.warn("File {} not found. {} not loaded.".format(fn, attr))
+        with open(os.path.join(directory, root_name + ".json")) as fp:
+            meta = json.load(fp)
+        self.perturbation_modulus = meta["data"]["perturbation_modulus"]
+        self.T = meta["T"]
+        self.ignore_harmonic = meta["ignore_v2"]
+        self.ignore_v3 = meta["ignore_v3"]
+        self.ignore_v4 = meta["ignore_v4"]
+        self.N = meta["data"]["n_configs"]
+        self.n_modes = meta["data"]["n_modes"]
+        self.reverse_L = meta["data"]["reverse"]
+        self.shift_value = meta["data"]["shift"]
 
     def _padded_symmetries(self):
         if self.symmetries is None:

@@ -67,6 +67,9 @@ SIGNATURES = {
                                                    ctypes.c_int]),
     "tdscha_lanczos_group_run": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "tdscha_lanczos_group_destroy": (ctypes.c_int, [_vp]),
+    "tdscha_project_ensemble": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+        _vp, _vp, _vp]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

@@ -18,6 +18,12 @@ Outputs (tests/golden/):
   lanczos_wigner.npz    the same for the Wigner symmetric run
   ref_kernel_vectors.npz (f_av, d2v) of the reference C trio (oracle/_ref, compiled from
                         CModules/LanczosFunctions.c) on mode10 for several (R1, Y1, D4) inputs
+  projection_snte.npz   raw ensemble of tests/test_lanczos_fast/data (xats_pop1.npy,
+                        forces_pop1.npy), the centroid positions rebuilt from dyn_gen_pop1_1,
+                        and the reference's projected X, Y with its m, pols, w, rho
+                        (normal/LANCZOS_STEP5.npz): the inputs and outputs of Py:380-442
+  input_files_mode10.npz / .json  masses of mode10 and the SHA-256 of every mode10.* text
+                        file that prepare_input_files (Py:862-919) wrote
 """
 import os
 import sys
@@ -133,8 +139,58 @@ def main():
         out["d2v_%d" % k] = d2v
     out["n_cases"] = np.int64(len(cases))
     np.savez_compressed(os.path.join(HERE, "ref_kernel_vectors.npz"), **out)
+    projection_fixture()
+    input_files_fixture()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+def centroid_from_dyn(path, supercell, bohr_to_angstrom):
+    """Supercell centroid positions (Angstrom) from a Quantum ESPRESSO dynamical-matrix file
+    with ibrav = 0: unit cell and atoms in units of alat (celldm(1), Bohr); supercell atoms
+    ordered cell by cell (third lattice index fastest), unit-cell atoms fastest."""
+    import itertools
+    lines = open(path).read().split("\n")
+    head = lines[2].split()
+    ntyp, nat, ibrav, alat = int(head[0]), int(head[1]), int(head[2]), float(head[3])
+    assert ibrav == 0
+    cell = np.array([[float(x) for x in lines[4 + i].split()] for i in range(3)])
+    pos = np.array([[float(x) for x in lines[7 + ntyp + a].split()[2:5]] for a in range(nat)])
+    cell = cell * alat * bohr_to_angstrom
+    pos = pos * alat * bohr_to_angstrom
+    out = [pos[a] + np.array(ijk, dtype=float) @ cell
+           for ijk in itertools.product(*[range(s) for s in supercell]) for a in range(nat)]
+    return np.array(out)
+
+
+def projection_fixture():
+    d = os.path.join(REF, "tests/test_lanczos_fast")
+    st = load_npz(os.path.join(d, "normal/LANCZOS_STEP5.npz"))
+    xats = np.load(os.path.join(d, "data/xats_pop1.npy"), allow_pickle=False)
+    forces = np.load(os.path.join(d, "data/forces_pop1.npy"), allow_pickle=False)
+    # CellConstructor's Bohr -> Angstrom constant for alat (the reference's X is matched to
+    # 8e-9 with it, 1.5e-6 with the CODATA value).
+    centroid = centroid_from_dyn(os.path.join(d, "data/dyn_gen_pop1_1"), (2, 2, 2), 0.529177249)
+    assert np.abs(xats - centroid[None]).max() < 0.5
+    np.savez_compressed(os.path.join(HERE, "projection_snte.npz"), xats=xats, forces=forces,
+                        centroid=centroid, m=st["m"], pols=st["pols"], w=st["w"], X=st["X"],
+                        Y=st["Y"], rho=st["rho"], T=np.float64(st["T"]))
+
+
+def input_files_fixture():
+    import hashlib
+    import json
+    d = os.path.join(REF, "tests/test_lanczos_fast/test_c_code_lanczos")
+    hashes = {}
+    for f in sorted(os.listdir(d)):
+        if f.startswith("mode10."):
+            hashes[f] = hashlib.sha256(open(os.path.join(d, f), "rb").read()).hexdigest()
+    with open(os.path.join(d, "mode10.json")) as fp:
+        meta = json.load(fp)
+    with open(os.path.join(HERE, "input_files_mode10.json"), "w") as fp:
+        json.dump({"sha256": hashes, "json": meta}, fp, indent=1, sort_keys=True)
+    np.savez_compressed(os.path.join(HERE, "input_files_mode10.npz"),
+                        masses=np.loadtxt(os.path.join(d, "mode10.masses")))
 
 
 if __name__ == "__main__":
