@@ -247,6 +247,20 @@ int tdscha_project_ensemble(int n_configs, int n_cart, int n_modes, const double
                             const double *f_mean, const double *masses, const double *pols,
                             double bohr, double *X, double *Y, void *stream);
 
+/* ---- Symmetry operations in the mode basis --------------------------------------------
+ * Replaces CellConstructor's GetSymmetriesOnModesDeg as Lanczos.prepare_symmetrization calls
+ * it (Modules/DynamicalLanczos.py:735-746): for every symmetry s and block b of degenerate
+ * modes [block_start[b], block_start[b+1]),
+ *   out_b[s][i][j] = sum_a sum_x pols[3 irt[s][a] + x][b0 + i] (rot[s] pols[a])[x][b0 + j].
+ * Device pointers: rot [n_sym x 3 x 3] Cartesian rotations, irt [n_sym x nat] the atom each
+ * atom is carried to (a permutation; entries outside [0, nat) are clamped, not reported),
+ * pols [3 nat x n_modes] mass-scaled polarization vectors, out the blocks one after the
+ * other, each [n_sym x d_b x d_b] row-major. block_start: HOST array of n_blocks + 1
+ * increasing offsets starting at 0 (blocks at most 170 modes wide). Enqueued on `stream`. */
+int tdscha_symmetries_on_modes(int n_sym, int nat, int n_modes, const double *rot,
+                               const int *irt, const double *pols, int n_blocks,
+                               const int *block_start, double *out, void *stream);
+
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;
  * query synchronises on them and returns the summed elapsed time and the launch count. */

@@ -23,6 +23,7 @@ Contents
   - biconjugate / symmetric Lanczos run_FT  Py:5843-6372
   - continued-fraction Green function       Py:4951-5047
   - ensemble projection on the modes        Py:380-442 (``project_ensemble``)
+  - symmetry operations on mode blocks      Py:735-746 (``symmetries_on_modes``)
 * The derivative of a linear functional of the kernel outputs with respect to the
   configuration forces Y (``force_grad``), image by image in the reference's own form
   (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
@@ -688,3 +689,22 @@ def project_ensemble(u_disps, forces, sscha_forces, f_mean, masses, pols, bohr):
     X = u.dot(pols)                                                     # Py:438
     Y = f.dot(pols)                                                     # Py:439
     return X, Y
+
+
+def symmetries_on_modes(rotations, irt, pols, block_start):
+    """S_b[s] = E_b^T P_s E_b with (P_s E)[irt_s(a)] = R_s E[a], per block of consecutive
+    modes [block_start[b], block_start[b+1]) (what prepare_symmetrization stores through
+    CellConstructor's GetSymmetriesOnModesDeg, Py:735-746)."""
+    rot = np.asarray(rotations, dtype=np.float64)
+    irt = np.asarray(irt)
+    nat = irt.shape[1]
+    E = np.asarray(pols, dtype=np.float64).reshape(nat, 3, -1)
+    out = []
+    for b0, b1 in zip(block_start[:-1], block_start[1:]):
+        Eb = E[:, :, b0:b1]
+        S = np.empty((len(rot), b1 - b0, b1 - b0))
+        for s in range(len(rot)):
+            RE = np.einsum("xy,ayj->axj", rot[s], Eb)
+            S[s] = np.einsum("axi,axj->ij", Eb[irt[s]], RE)
+        out.append(S)
+    return out

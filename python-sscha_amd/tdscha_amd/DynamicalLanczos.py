@@ -162,6 +162,40 @@ class Lanczos(object):
                                    device=torch.device("cuda", self.device))
         self.set_ensemble(X.cpu().numpy(), Y.cpu().numpy(), w_sel, rho, T, pols_sel, m)
 
+    def prepare_symmetrization(self, no_sym=False, verbose=True, symmetries=None,
+                               deg_tol=1e-8):
+        """Mode-basis symmetry blocks built on the device (Py:686-778).
+
+        symmetries: (rotations, irt) of the supercell's space group, Cartesian 3 x 3 rotations
+        (n_sym, 3, 3) and the atom each operation carries every atom to (n_sym, nat) (see
+        symmetry.atom_permutations); the reference asks spglib for them, which is not
+        available here. no_sym: one 1 x 1 identity block per mode (Py:717-722). Blocks are the
+        runs of consecutive modes whose frequencies agree to deg_tol (relative)."""
+        from . import symmetry as sy
+
+        n = self.n_modes
+        if no_sym:
+            self.set_symmetries([np.ones((1, 1, 1), dtype=TYPE_DP)] * n,
+                                np.ones(n, dtype=np.intc),
+                                [np.array([i], dtype=np.intc) for i in range(n)],
+                                np.arange(n).astype(np.intc))
+            return
+        if symmetries is None:
+            raise ValueError("pass symmetries=(rotations, irt): the space group is not searched "
+                             "here (the reference uses spglib)")
+        pols = np.asarray(self.pols)
+        if pols.ndim != 2 or pols.shape[1] != n or pols.shape[0] % 3:
+            raise ValueError("the polarization vectors of the kept modes are not set")
+        rot, irt = symmetries
+        t0 = time.time()
+        starts = sy.degenerate_blocks(self.w, deg_tol)
+        blocks = sy.symmetries_on_modes(rot, irt, pols, starts, device=self.device)
+        ndeg, degs, bid = sy.block_arrays(starts, n)
+        self.set_symmetries(blocks, ndeg, degs, bid)
+        if verbose:
+            print("Time to convert symmetries in the polarization space: {} s".format(
+                time.time() - t0))
+
     def set_symmetries(self, symmetries, N_degeneracy, degenerate_space, block_id):
         """Mode-basis symmetry blocks (what prepare_symmetrization stores, Py:741-778)."""
         self.symmetries = [np.ascontiguousarray(s, dtype=TYPE_DP) for s in symmetries]

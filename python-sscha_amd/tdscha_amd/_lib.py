@@ -70,6 +70,8 @@ SIGNATURES = {
     "tdscha_project_ensemble": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
         _vp, _vp, _vp]),
+    "tdscha_symmetries_on_modes": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _ip, _vp, _vp]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

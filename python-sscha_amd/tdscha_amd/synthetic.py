@@ -267,14 +267,12 @@ def modes(cr: Crystal, phi, tol=1e-7):
     return w, E, blocks
 
 
-def symmetry_blocks(cr: Crystal, E, blocks, chunk=256, device=None):
-    """S_g restricted to the degenerate blocks for g in Oh x translations."""
-    import torch
-    dev = device if device is not None else torch.device("cpu")
+def space_group(cr: Crystal):
+    """Cartesian rotations (n_sym, 3, 3) and atom permutations (n_sym, nat) of the
+    Oh x translations operations of the supercell."""
     ops = oh_operations()
     trans = cr.cell[cr.species == 0] @ PRIM   # lattice translations in a/2 units, one per cell
     nat = cr.nat
-    # atom permutation for every (op, translation)
     perms = np.empty((len(ops) * len(trans), nat), dtype=np.int64)
     rots = np.empty((len(ops) * len(trans), 3, 3))
     g = 0
@@ -284,8 +282,21 @@ def symmetry_blocks(cr: Crystal, E, blocks, chunk=256, device=None):
             perms[g] = cr.index_of(rp + t, cr.species)
             rots[g] = R
             g += 1
-    n_sym = g
+    return rots, perms
+
+
+def symmetry_blocks(cr: Crystal, E, blocks, chunk=256, device=None):
+    """S_g restricted to the degenerate blocks for g in Oh x translations (on a GPU device
+    with the library's kernel, tdscha_symmetries_on_modes; on the CPU with torch)."""
+    import torch
+    dev = device if device is not None else torch.device("cpu")
+    rots, perms = space_group(cr)
+    n_sym, nat = perms.shape
     n = E.shape[1]
+    if dev.type == "cuda":
+        from . import symmetry as sym
+        starts = np.array([b0 for b0, _ in blocks] + [blocks[-1][1]], dtype=np.intc)
+        return sym.symmetries_on_modes(rots, perms, E, starts, device=dev), perms, rots
     Et = torch.as_tensor(E.reshape(nat, 3, n), device=dev)
     out = [np.empty((n_sym, b1 - b0, b1 - b0)) for b0, b1 in blocks]
     chunk = max(1, min(chunk, int(2e8 // (nat * 3 * n))))
