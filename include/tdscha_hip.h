@@ -261,6 +261,16 @@ int tdscha_symmetries_on_modes(int n_sym, int nat, int n_modes, const double *ro
                                const int *irt, const double *pols, int n_blocks,
                                const int *block_start, double *out, void *stream);
 
+/* ---- Weighted ensemble average (perturbation builders) ----------------------------------
+ * C[m][n] = scale * sum_k A[k][m] w[k] B[k][n]: the contraction over configurations behind
+ * Modules/Perturbations.py get_d1M_dR_av (:195), get_d1alpha_dR_av (:240), get_d2M_dR_av
+ * (:275) and get_d2alpha_dR_av (:330), and the Upsilon product that follows the last two.
+ * Device pointers, FP64, row-major with leading dimensions lda >= M, ldb >= N, ldc >= N;
+ * w may be NULL (all ones). Enqueued on `stream`. */
+int tdscha_weighted_tn_product(int K, int M, int N, const double *A, int lda, const double *w,
+                               const double *B, int ldb, double scale, double *C, int ldc,
+                               void *stream);
+
 /* ---- Event timing of the kernel groups (no reference counterpart: measurement only) ---
  * When enabled, every launch of a group is bracketed by two HIP events on its stream;
  * query synchronises on them and returns the summed elapsed time and the launch count. */

@@ -24,6 +24,8 @@ Contents
   - continued-fraction Green function       Py:4951-5047
   - ensemble projection on the modes        Py:380-442 (``project_ensemble``)
   - symmetry operations on mode blocks      Py:735-746 (``symmetries_on_modes``)
+  - ensemble averages of effective charges and Raman tensors, Modules/Perturbations.py
+    :195-375 (``d1M_dR_av``, ``d1alpha_dR_av``, ``d2M_dR_av``, ``d2alpha_dR_av``)
 * The derivative of a linear functional of the kernel outputs with respect to the
   configuration forces Y (``force_grad``), image by image in the reference's own form
   (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
@@ -708,3 +710,29 @@ def symmetries_on_modes(rotations, irt, pols, block_start):
             S[s] = np.einsum("axi,axj->ij", Eb[irt[s]], RE)
         out.append(S)
     return out
+
+
+def d1M_dR_av(rho, Z):
+    """Perturbations.py:195-238: Z (N, nat, 3 field, 3 cart) -> (3 nat, 3)."""
+    N = len(rho)
+    new = np.array([np.einsum("abc ->bac", Z[i]).reshape((3, -1)).T for i in range(N)])
+    return np.einsum("iab, i", new, rho) / np.sum(rho)
+
+
+def d1alpha_dR_av(rho, R):
+    """Perturbations.py:240-273: R (N, 3, 3, 3 nat) -> (3, 3, 3 nat)."""
+    return np.einsum("iabc, i", R, rho) / np.sum(rho)
+
+
+def d2M_dR_av(u_bohr, rho, Z, ups):
+    """Perturbations.py:275-328 with u already in Bohr: (3 nat, 3 nat, 3)."""
+    N = len(rho)
+    v = np.einsum("ab, ib -> ia", ups, u_bohr)
+    new = np.array([np.einsum("abc ->bac", Z[i]).reshape((3, -1)).T for i in range(N)])
+    return np.einsum("i, ia, ibc -> abc", rho, v, new) / np.sum(rho)
+
+
+def d2alpha_dR_av(u_bohr, rho, R, ups):
+    """Perturbations.py:330-375 with u already in Bohr: (3, 3, 3 nat, 3 nat)."""
+    v = np.einsum("ab, ib -> ia", ups, u_bohr)
+    return np.einsum("i, ia, ibcd -> bcda", rho, v, R) / np.sum(rho)

@@ -98,6 +98,8 @@ class Lanczos(object):
         self.perturbation_modulus = 1
         self.use_wigner = use_wigner
         self.ignore_small_w = False
+        self.effective_charges = None   # (nat_uc, 3, 3), read by prepare_ir
+        self.raman_tensor = None        # (3, 3, 3 nat_uc), read by prepare_raman
         self.L_linop = None
         self._dev_ens = None
         if ensemble is not None and hasattr(ensemble, "u_disps"):
@@ -264,6 +266,88 @@ class Lanczos(object):
         new_v = np.einsum("a, a, ab->b", m_on, vector, self.pols)
         self.psi[:self.n_modes] += new_v
         self.perturbation_modulus = new_v.dot(new_v)
+
+    def _n_supercell(self, n_cart_uc):
+        n_cart = len(self.m)
+        if n_cart == 0 or n_cart % n_cart_uc:
+            raise ValueError("the masses (3 nat of the supercell) are not set or do not match "
+                             "the tensor's unit cell")
+        return n_cart // n_cart_uc
+
+    def prepare_ir(self, effective_charges=None, pol_vec=np.array([1, 0, 0])):
+        """IR perturbation (Py:1647-1691): Z* contracted with the light polarization, tiled
+        over the supercell, divided by sqrt(m) and projected on the modes. effective_charges
+        (nat_uc, 3 field, 3 Cartesian); defaults to self.effective_charges."""
+        from . import perturbations as pt
+        ec = self.effective_charges if effective_charges is None else effective_charges
+        if ec is None:
+            raise ValueError("Error, no effective charge found. Cannot initialize IR responce")
+        ec = np.asarray(ec, dtype=TYPE_DP)
+        if ec.ndim != 3 or ec.shape[1] != 3 or ec.shape[2] != 3:
+            raise ValueError("Error, effective charges of the wrong shape: {}".format(ec.shape))
+        n_sc = self._n_supercell(3 * ec.shape[0])
+        if not self.ignore_small_w and ec.shape[0] * ec.shape[2] * n_sc != self.n_modes + 3:
+            raise ValueError("effective charges do not match the modes (n_modes + 3 = %d)"
+                             % (self.n_modes + 3))
+        self.prepare_perturbation(np.tile(pt.ir_vector(ec, pol_vec), n_sc), masses_exp=-1)
+
+    def prepare_raman(self, pol_vec_in=np.array([1, 0, 0]), pol_vec_out=np.array([1, 0, 0]),
+                      mixed=False, pol_in_2=None, pol_out_2=None, unpolarized=None,
+                      raman_tensor=None):
+        """Raman perturbation (Py:1003-1124). raman_tensor (3, 3, 3 nat_uc); defaults to
+        self.raman_tensor. mixed adds the (pol_in_2, pol_out_2) component. unpolarized = 0..6
+        selects the components of the unpolarized signal (alpha^2 from xx + yy + zz over 3;
+        (xx - yy) / sqrt 2, (xx - zz) / sqrt 2, (yy - zz) / sqrt 2; sqrt 3 xy, sqrt 3 yz,
+        sqrt 3 xz).
+
+        Deliberate fixes: the reference computes the mixed vector and then overwrites it
+        (Py:1037-1049), and after the unpolarized branches it prepares the last component
+        alone again (Py:1123), discarding the sum built with add=True; here the mixed vector
+        and the whole unpolarized sum are what is prepared."""
+        from . import perturbations as pt
+        T = self.raman_tensor if raman_tensor is None else raman_tensor
+        if T is None:
+            raise ValueError("Error, no Raman tensor found. Cannot initialize the Raman responce")
+        T = np.asarray(T, dtype=TYPE_DP)
+        n_sc = self._n_supercell(T.shape[2])
+        x, y, z = np.eye(3)
+        if unpolarized is None:
+            v = pt.raman_vector(T, pol_vec_in, pol_vec_out)
+            if mixed:
+                v = v + pt.raman_vector(T, pol_in_2, pol_out_2)
+            self.prepare_perturbation(np.tile(v, n_sc), masses_exp=-1)
+            return
+        r2, r3 = 1 / np.sqrt(2), np.sqrt(3)
+        parts = {0: ((x, x, 1 / 3), (y, y, 1 / 3), (z, z, 1 / 3)),
+                 1: ((x, x, r2), (y, y, -r2)), 2: ((x, x, r2), (z, z, -r2)),
+                 3: ((y, y, r2), (z, z, -r2)), 4: ((x, y, r3),), 5: ((y, z, r3),),
+                 6: ((x, z, r3),)}
+        if unpolarized not in parts:
+            raise ValueError("Error, unpolarized must be between [0, ... ,6] got invalid {}."
+                             .format(unpolarized))
+        for k, (p_in, p_out, c) in enumerate(parts[unpolarized]):
+            v = np.tile(pt.raman_vector(T, p_in, p_out), n_sc) * c
+            self.prepare_perturbation(v, masses_exp=-1, add=k > 0)
+
+    def get_prefactors_unpolarized_raman(self, index):
+        """Py:1126-1150."""
+        from . import perturbations as pt
+        if index not in range(7):
+            raise ValueError("{} should be in {}".format(index, list(range(7))))
+        return pt.UNPOLARIZED_RAMAN_PREFACTORS[index]
+
+    def prepare_unpolarized_raman(self, index=0, debug=False, raman_tensor=None):
+        """Py:1152-1221: component `index` of the unpolarized Raman signal (see
+        perturbations.unpolarized_raman_vector), tiled over the supercell, / sqrt(m)."""
+        from . import perturbations as pt
+        T = self.raman_tensor if raman_tensor is None else raman_tensor
+        if T is None:
+            raise ValueError("Error, no Raman tensor found. Cannot initialize the Raman responce")
+        T = np.asarray(T, dtype=TYPE_DP)
+        v = pt.unpolarized_raman_vector(T, index)
+        if debug:
+            np.save("raman_v_{}".format(index), v)
+        self.prepare_perturbation(np.tile(v, self._n_supercell(T.shape[2])), masses_exp=-1)
 
     # ------------------------------------------------------------------ psi helpers
     def get_Y1(self, half_off_diagonal=False):

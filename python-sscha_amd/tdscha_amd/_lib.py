@@ -72,6 +72,9 @@ SIGNATURES = {
         _vp, _vp, _vp]),
     "tdscha_symmetries_on_modes": (ctypes.c_int, [
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _ip, _vp, _vp]),
+    "tdscha_weighted_tn_product": (ctypes.c_int, [
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
+        ctypes.c_double, _vp, ctypes.c_int, _vp]),
     "tdscha_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "tdscha_profile_reset": (ctypes.c_int, []),
     "tdscha_profile_query": (ctypes.c_int, [ctypes.c_int, _dp,

@@ -24,6 +24,8 @@ Outputs (tests/golden/):
                         (normal/LANCZOS_STEP5.npz): the inputs and outputs of Py:380-442
   input_files_mode10.npz / .json  masses of mode10 and the SHA-256 of every mode10.* text
                         file that prepare_input_files (Py:862-919) wrote
+  raman_unpol.npz       tests/test_raman_unpol: the Raman tensor and the seven unpolarized
+                        Raman vectors its test checks (raman_tensor_eq.npy, raman_v_*.npy)
 """
 import os
 import sys
@@ -141,6 +143,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "ref_kernel_vectors.npz"), **out)
     projection_fixture()
     input_files_fixture()
+    raman_fixture()
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 
@@ -191,6 +194,16 @@ def input_files_fixture():
         json.dump({"sha256": hashes, "json": meta}, fp, indent=1, sort_keys=True)
     np.savez_compressed(os.path.join(HERE, "input_files_mode10.npz"),
                         masses=np.loadtxt(os.path.join(d, "mode10.masses")))
+
+
+
+
+def raman_fixture():
+    d = os.path.join(REF, "tests/test_raman_unpol")
+    out = {"raman_tensor": np.load(os.path.join(d, "raman_tensor_eq.npy"), allow_pickle=False)}
+    for i in range(7):
+        out["raman_v_%d" % i] = np.load(os.path.join(d, "raman_v_%d.npy" % i), allow_pickle=False)
+    np.savez_compressed(os.path.join(HERE, "raman_unpol.npz"), **out)
 
 
 if __name__ == "__main__":
