@@ -115,21 +115,51 @@ template <int G> struct GrpTile {
   static constexpr int TPW = (G / 16) * (G / 16) / 4;   // 16 x 16 sub-tiles per wave
 };
 
-// Dense S_g of symmetry Ss (zero outside g's blocks and outside the group width).
+// Dense S_g of one symmetry (zero outside g's blocks and outside the group width), staged
+// software-pipelined: each of the 256 threads owns G^2 / 256 elements
+// (element idx = thread + 256 q); their offsets inside one symmetry's packed blocks do not
+// depend on the symmetry and are found once (-1: the element is zero), so that the values of
+// the next slot can be loaded into registers, branch-free, while the current slot is
+// multiplied.
+template <int G> struct GrpSIdx {
+  static constexpr int PT = G * G / 256;
+  int off[PT];
+};
+
 template <int G>
-__device__ inline void build_group_s(double *Sl, const double *__restrict__ Ss, int2 grp,
-                                     const ModeMeta *__restrict__ meta) {
-  constexpr int LD = GrpTile<G>::LD;
-  for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
-    const int r = idx / G, c = idx - r * G;
-    double v = 0.0;
+__device__ inline void group_s_index(int2 grp, const ModeMeta *__restrict__ meta,
+                                     GrpSIdx<G> &ix) {
+#pragma unroll
+  for (int q = 0; q < GrpSIdx<G>::PT; ++q) {
+    const int idx = threadIdx.x + 256 * q, r = idx / G, c = idx - r * G;
+    int o = -1;
     if (r < grp.y && c < grp.y) {
       const ModeMeta mr = meta[grp.x + r];
-      const int lb = mr.blk_off - grp.x, cc = c - lb;
-      if (cc >= 0 && cc < mr.blk_dim)
-        v = Ss[mr.sym_off + (grp.x + r - mr.blk_off) * mr.blk_dim + cc];
+      const int cc = c - (mr.blk_off - grp.x);
+      if (cc >= 0 && cc < mr.blk_dim) o = mr.sym_off + (grp.x + r - mr.blk_off) * mr.blk_dim + cc;
     }
-    Sl[r * LD + c] = v;
+    ix.off[q] = o;
+  }
+}
+
+template <int G>
+__device__ inline void group_s_load(const double *__restrict__ Ss, const GrpSIdx<G> &ix,
+                                    double (&v)[GrpSIdx<G>::PT]) {
+#pragma unroll
+  for (int q = 0; q < GrpSIdx<G>::PT; ++q) {
+    const int o = ix.off[q];
+    const double x = Ss[o < 0 ? 0 : o];
+    v[q] = o < 0 ? 0.0 : x;
+  }
+}
+
+template <int G>
+__device__ inline void group_s_store(double *Sl, const double (&v)[GrpSIdx<G>::PT]) {
+  constexpr int LD = GrpTile<G>::LD;
+#pragma unroll
+  for (int q = 0; q < GrpSIdx<G>::PT; ++q) {
+    const int idx = threadIdx.x + 256 * q, r = idx / G, c = idx - r * G;
+    Sl[r * LD + c] = v[q];
   }
 }
 
@@ -204,12 +234,25 @@ __global__ __launch_bounds__(256) void k_rot1_grp(const double *__restrict__ Ysy
     const int r = idx / G, c = idx - r * G;
     M[r * LD + c] = (r < gg.y && c < gh.y) ? Yc[(size_t)(gg.x + r) * npd + gh.x + c] : 0.0;
   }
+  GrpSIdx<G> ig, ih;
+  group_s_index<G>(gg, meta, ig);
+  group_s_index<G>(gh, meta, ih);
+  double sg[GrpSIdx<G>::PT], sh[GrpSIdx<G>::PT];
+  if (blockIdx.y < ps.ns) {
+    const double *Ss = S + (size_t)ps.sym(ch * ps.ns + blockIdx.y) * sym_stride;
+    group_s_load<G>(Ss, ig, sg);
+    group_s_load<G>(Ss, ih, sh);
+  }
   for (int t = blockIdx.y; t < ps.ns; t += gridDim.y) {
     const int sl = ch * ps.ns + t;
-    const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
     __syncthreads();   // previous slot's reads of Sg / Sh / T are done
-    build_group_s<G>(Sg, Ss, gg, meta);
-    build_group_s<G>(Sh, Ss, gh, meta);
+    group_s_store<G>(Sg, sg);
+    group_s_store<G>(Sh, sh);
+    if (t + (int)gridDim.y < ps.ns) {   // the next slot's blocks, in flight meanwhile
+      const double *Sn = S + (size_t)ps.sym(sl + gridDim.y) * sym_stride;
+      group_s_load<G>(Sn, ig, sg);
+      group_s_load<G>(Sn, ih, sh);
+    }
     __syncthreads();
     v4d acc[TPW];
 #pragma unroll
@@ -270,22 +313,41 @@ __global__ __launch_bounds__(256) void k_rot2_grp(const TB *__restrict__ B,
       acc[q][r] = (i < gg.y && j < gh.y) ? d2v[(size_t)(gg.x + i) * npd + gh.x + j] : 0.0;
     }
   }
-  for (int t = 0; t < ps.ns; ++t) {
+  // the slot's B[g][h], B[h][g] and symmetry blocks are loaded into registers (clamped,
+  // branch-free) one slot ahead, while the previous slot is multiplied
+  constexpr int PT = GrpSIdx<G>::PT;
+  GrpSIdx<G> ig, ih;
+  group_s_index<G>(gg, meta, ig);
+  group_s_index<G>(gh, meta, ih);
+  TB bm[PT], bt[PT];
+  double sg[PT], sh[PT];
+  auto load_slot = [&](int t) {
     const int sl = ch * ps.ns + t;
     const double *Ss = S + (size_t)ps.sym(sl) * sym_stride;
     const TB *Bs = B + (size_t)sl * npd * npd;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int idx = threadIdx.x + 256 * q, r = idx / G, c = idx - r * G;
+      bm[q] = Bs[(size_t)min(gg.x + r, npd - 1) * npd + min(gh.x + c, npd - 1)];
+      bt[q] = Bs[(size_t)min(gh.x + r, npd - 1) * npd + min(gg.x + c, npd - 1)];
+    }
+    group_s_load<G>(Ss, ig, sg);
+    group_s_load<G>(Ss, ih, sh);
+  };
+  if (ps.ns > 0) load_slot(0);
+  for (int t = 0; t < ps.ns; ++t) {
     __syncthreads();
     // M = B[g][h] + (B[h][g])^T: both tiles read along their rows (coalesced), the second
     // staged in T and added transposed
-    for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
-      const int r = idx / G, c = idx - r * G;
-      M[r * LD + c] = (r < gg.y && c < gh.y) ? (double)Bs[(size_t)(gg.x + r) * npd + gh.x + c]
-                                             : 0.0;
-      T[r * LD + c] = (r < gh.y && c < gg.y) ? (double)Bs[(size_t)(gh.x + r) * npd + gg.x + c]
-                                             : 0.0;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int idx = threadIdx.x + 256 * q, r = idx / G, c = idx - r * G;
+      M[r * LD + c] = (r < gg.y && c < gh.y) ? (double)bm[q] : 0.0;
+      T[r * LD + c] = (r < gh.y && c < gg.y) ? (double)bt[q] : 0.0;
     }
-    build_group_s<G>(Sg, Ss, gg, meta);
-    build_group_s<G>(Sh, Ss, gh, meta);
+    group_s_store<G>(Sg, sg);
+    group_s_store<G>(Sh, sh);
+    if (t + 1 < ps.ns) load_slot(t + 1);
     __syncthreads();
     for (int idx = threadIdx.x; idx < G * G; idx += blockDim.x) {
       const int r = idx / G, c = idx - r * G;
