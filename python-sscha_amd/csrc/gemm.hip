@@ -294,11 +294,15 @@ __global__ __launch_bounds__(256) void k_gemm2(Gemm2Args<T> g) {
 
 // ---------------------------------------------------------------- f_av back-projection
 // V[s][j] = sum_k YT[j][k] e1_s[k] + XtT[j][k] e2_s[k] in FP64 MFMA for every precision:
-// an NT product (rows j x slots s, k contiguous in both operands), 64 rows x 32 slots per
-// workgroup (waves 2 x 2, each 32 rows x 16 slots), split over k in chunks of kFavChunk
-// configurations; the partial sums Vp[chunk][s][j] are added in a fixed order afterwards.
-constexpr int FV_S = 32;
+// an NT product (rows j x slots s, k contiguous in both operands), 64 rows x 96 slots per
+// workgroup (waves 2 x 2, each 32 rows x 48 slots; a pass's slots in one workgroup column,
+// so YT and XtT are read once per pass), split over k in chunks of kFavChunk configurations;
+// the partial sums Vp[chunk][s][j] are added in a fixed order afterwards. The weight loads
+// are clamped and branch-free (zeroed when staged), so the next slice stays in flight under
+// the MFMAs.
+constexpr int FV_S = 96;
 constexpr int FV_LDB = FV_S + 16;
+constexpr int FV_PT = FV_S * BK / 256;   // weights per thread and slice
 
 __global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
                                                   const double *__restrict__ XtT,
@@ -314,33 +318,31 @@ __global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
   const int k_begin = kc * kFavChunk;
   const int k_end = min(Np, k_begin + kFavChunk);
   const int KT = (k_end - k_begin) / BK;
-  // B slice: 32 slots x 16 k, two consecutive k per thread
-  const int bq = threadIdx.x >> 3, bk = (threadIdx.x & 7) * 2;
-  const bool bq_ok = q0 + bq < sb;
+  // B slice: 96 slots x 16 k; thread t stages k = t % 16 of slots t / 16 + 16 p (p < 6)
+  const int bk = threadIdx.x & 15, bq = threadIdx.x >> 4;
 
-  v4d acc[2];
+  v4d acc[2][3];
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[ti][r] = 0.0;
+    for (int tj = 0; tj < 3; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[ti][tj][r] = 0.0;
 
   for (int part = 0; part < 2; ++part) {
     const double *M = part ? XtT : YT;
     const double *E = W + (size_t)(part ? 3 : 2) * Np;   // e2 or e1 of slot 0
-    auto load_b = [&](int k0, double (&rb)[2]) {
-      if (bq_ok) {
-        const double *p = E + (size_t)(q0 + bq) * 4 * Np + k0 + bk;
-        rb[0] = p[0];
-        rb[1] = p[1];
-      } else {
-        rb[0] = rb[1] = 0.0;
-      }
+    auto load_b = [&](int k0, double (&rb)[FV_PT]) {
+#pragma unroll
+      for (int p = 0; p < FV_PT; ++p)
+        rb[p] = E[(size_t)min(q0 + bq + 16 * p, sb - 1) * 4 * Np + k0 + bk];
     };
-    auto store_b = [&](double *lds, const double (&rb)[2]) {
-      lds[bk * FV_LDB + bq] = rb[0];
-      lds[(bk + 1) * FV_LDB + bq] = rb[1];
+    auto store_b = [&](double *lds, const double (&rb)[FV_PT]) {
+#pragma unroll
+      for (int p = 0; p < FV_PT; ++p)
+        lds[bk * FV_LDB + bq + 16 * p] = q0 + bq + 16 * p < sb ? rb[p] : 0.0;
     };
-    double ra[4], rb[2];
+    double ra[4], rb[FV_PT];
     load_slice(M, Np, j0, k_begin, ra);
     load_b(k_begin, rb);
     __syncthreads();   // the previous part's last reads of As / Bs are done
@@ -358,9 +360,12 @@ __global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
         const int k = kq * 4 + lg;
         const double a0 = As[cur][k * LDS_LD + wr * 32 + lr];
         const double a1 = As[cur][k * LDS_LD + wr * 32 + 16 + lr];
-        const double b = Bs[cur][k * FV_LDB + wc * 16 + lr];
-        acc[0] = mfma(a0, b, acc[0]);
-        acc[1] = mfma(a1, b, acc[1]);
+#pragma unroll
+        for (int tj = 0; tj < 3; ++tj) {
+          const double b = Bs[cur][k * FV_LDB + wc * 48 + tj * 16 + lr];
+          acc[0][tj] = mfma(a0, b, acc[0][tj]);
+          acc[1][tj] = mfma(a1, b, acc[1][tj]);
+        }
       }
       if (kt + 1 < KT) {
         store_slice(As[cur ^ 1], ra);
@@ -369,13 +374,17 @@ __global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
       __syncthreads();
     }
   }
-  const int q = q0 + wc * 16 + lr;
-  if (q < sb) {
-    double *out = Vp + ((size_t)kc * sb + q) * npd + j0;
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
+  for (int tj = 0; tj < 3; ++tj) {
+    const int q = q0 + wc * 48 + tj * 16 + lr;
+    if (q < sb) {
+      double *out = Vp + ((size_t)kc * sb + q) * npd + j0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out[wr * 32 + ti * 16 + out_row<double>(lg, r)] = acc[ti][r];
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          out[wr * 32 + ti * 16 + out_row<double>(lg, r)] = acc[ti][tj][r];
+    }
   }
 }
 

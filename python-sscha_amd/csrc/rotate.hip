@@ -400,25 +400,32 @@ __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G
 }
 
 // f_acc[chain][a] += sum over the chain's slots of sum_k S_s[a'][k] V[slot][off_A + k]
-__global__ void k_fav_rot_acc(const double *__restrict__ V, double *__restrict__ f_acc,
-                              const double *__restrict__ S, const ModeMeta *__restrict__ meta, int n,
-                              int npd, int sym_stride, Pass ps) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
+// f_acc[c][a] += sum over the chain's slots t (in order) of (S_s^T V_s)[a]: the products are
+// formed for all (a, slot) at once into P (one thread each), then summed per chain in slot
+// order, so the result is the one of a sequential loop over the slots.
+__global__ void k_fav_rot_part(const double *__restrict__ V, double *__restrict__ P,
+                               const double *__restrict__ S, const ModeMeta *__restrict__ meta,
+                               int n, int npd, int sym_stride, Pass ps) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sl = blockIdx.y;
+  if (a >= n) return;
+  const ModeMeta ma = meta[a];
+  const int ap = a - ma.blk_off, d = ma.blk_dim;
+  const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off + ap * d;
+  const double *v = V + (size_t)sl * npd + ma.blk_off;
+  double part = 0.0;
+  for (int k = 0; k < d; ++k) part = fma(Sa[k], v[k], part);
+  P[(size_t)sl * npd + a] = part;
+}
+
+__global__ void k_fav_rot_sum(const double *__restrict__ P, double *__restrict__ f_acc, int n,
+                              int npd, Pass ps) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
   if (a >= n) return;
-  f_acc += (size_t)c * npd;
-  ModeMeta ma = meta[a];
-  const int ap = a - ma.blk_off, d = ma.blk_dim;
-  double acc = f_acc[a];
-  for (int t = 0; t < ps.ns; ++t) {
-    const int sl = c * ps.ns + t;
-    const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off + ap * d;
-    const double *v = V + (size_t)sl * npd + ma.blk_off;
-    double part = 0.0;
-    for (int k = 0; k < d; ++k) part = fma(Sa[k], v[k], part);
-    acc += part;
-  }
-  f_acc[a] = acc;
+  double acc = f_acc[(size_t)c * npd + a];
+  for (int t = 0; t < ps.ns; ++t) acc += P[(size_t)(c * ps.ns + t) * npd + a];
+  f_acc[(size_t)c * npd + a] = acc;
 }
 
 __global__ void k_zero(double *p, size_t count) {
@@ -563,8 +570,11 @@ void launch_rot_vec_from(const Ensemble &e, const Pass &ps, const double *in, do
 }
 
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st) {
-  hipLaunchKernelGGL(k_fav_rot_acc, dim3((e.n + 255) / 256, ps.nc), dim3(256), 0, st, e.V, e.f_acc,
-                     e.S, e.meta, e.n, e.npd, e.sym_stride, ps);
+  // the split-k partials of V are free once k_fav_reduce has summed them
+  hipLaunchKernelGGL(k_fav_rot_part, dim3((e.n + 255) / 256, ps.slots()), dim3(256), 0, st, e.V,
+                     e.Vp, e.S, e.meta, e.n, e.npd, e.sym_stride, ps);
+  hipLaunchKernelGGL(k_fav_rot_sum, dim3((e.n + 255) / 256, ps.nc), dim3(256), 0, st, e.Vp,
+                     e.f_acc, e.n, e.npd, ps);
 }
 
 void launch_zero(double *p, size_t count, hipStream_t st) {
