@@ -12,7 +12,11 @@ autotuner picks the contraction tile shape and the image slots per pass for this
 (timed on two passes per candidate, outside the timed region).
 
 Prints one JSON line. `roofline` is measured live with HIP events around every launch of
-each kernel group over the timed steps; `cpu_baseline` times the reference's own C trio
+each kernel group over the timed steps, for the contraction with the larger share of the
+time; `roofline_other_contraction` gives the other one. `frac` counts algorithmic flops
+(FLOP_L = 6 N N_sym n^2, GEMM1 2/6 of it, GEMM2 4/6); `frac_executed` the flops the kernel
+executes (GEMM2 forms B_s once and uses B_s + B_s^T, half its algorithmic count).
+`cpu_baseline` times the reference's own C trio
 (oracle/_ref, built from CModules/LanczosFunctions.c) on one host core over a bounded
 sample of images of the same ensemble and scales it linearly to the full image count.
 """
@@ -204,21 +208,30 @@ def main():
     flop_l = 6.0 * M * n * n
     # algorithmic flops per launch: GEMM1 carries the quadratic forms (2 N n^2 per symmetry),
     # GEMM2 the back-accumulation (4 N n^2 per symmetry)
+    # (GEMM2 forms B_s once and uses B_s + B_s^T: it executes half its algorithmic flops)
     share = {"gemm1": 2.0 / 6.0, "gemm2": 4.0 / 6.0}
-    dom = max(("gemm1", "gemm2"), key=lambda k: groups[k][0])
-    ms, cnt = groups[dom]
-    per_launch_flop = flop_l * args.chains * share[dom] / (cnt / args.steps)
-    achieved = per_launch_flop / (ms / cnt * 1e-3) / 1e12
+    executed = {"gemm1": 1.0, "gemm2": 0.5}
     peak = PEAK_TFLOPS[args.precision]
-    if args.precision == "bf16x3":
-        kernel = BF16_KERNELS[dens.tuning()[0]][0 if dom == "gemm1" else 1]
-    else:
-        kernel = "k_%s<%s>" % (dom, "double" if args.precision == "fp64" else "float")
     slots_run = (dens.tuning()[1] // args.chains) * args.chains
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+
+    def kernel_roofline(grp):
+        ms, cnt = groups[grp]
+        per_launch_flop = flop_l * args.chains * share[grp] / (cnt / args.steps)
+        achieved = per_launch_flop / (ms / cnt * 1e-3) / 1e12
+        if args.precision == "bf16x3":
+            kernel = BF16_KERNELS[dens.tuning()[0]][0 if grp == "gemm1" else 1]
+        else:
+            kernel = "k_%s<%s>" % (grp, "double" if args.precision == "fp64" else "float")
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                 "traffic": measured_traffic(kernel, args.workload, slots_run),
-                "kernel": kernel, "slots_per_launch": slots_run}
+                "kernel": kernel, "slots_per_launch": slots_run,
+                "ms_per_launch": round(ms / cnt, 3),
+                "frac_executed": round(achieved * executed[grp] / peak, 4)}
+
+    dom = max(("gemm1", "gemm2"), key=lambda k: groups[k][0])
+    roofline = kernel_roofline(dom)
+    roofline_other = kernel_roofline("gemm2" if dom == "gemm1" else "gemm1")
     value = args.chains * args.steps / elapsed
     fp64_ref = None
     if args.precision != "fp64" and args.fp64_sample_syms != 0:
@@ -272,6 +285,7 @@ def main():
                    "flop_per_L_application": flop_l,
                    "achieved_TFLOPs_whole_L": round(flop_l * value / 1e12, 3)},
         "roofline": roofline,
+        "roofline_other_contraction": roofline_other,
         "cpu_baseline": base,
         "fp64_plain": fp64_ref,
     }
