@@ -116,7 +116,7 @@ __device__ inline void write_weights(double *W, int Np, int i, double rho, doubl
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gemm1(Gemm1Args<T> g) {
+__global__ __launch_bounds__(256, 2) void k_gemm1(Gemm1Args<T> g) {
   typedef typename Acc<T>::type V;
   __shared__ T As[2][BK * LDS_LD];
   __shared__ T Bs[2][BK * LDS_LD];
@@ -239,7 +239,7 @@ __device__ inline void load_g_slice(const Gemm2Args<T> &g, const double *c1, con
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_gemm2(Gemm2Args<T> g) {
+__global__ __launch_bounds__(256, 2) void k_gemm2(Gemm2Args<T> g) {
   typedef typename Acc<T>::type V;
   __shared__ T As[2][BK * LDS_LD];
   __shared__ T Bs[2][BK * LDS_LD];
@@ -304,7 +304,7 @@ constexpr int FV_S = 96;
 constexpr int FV_LDB = FV_S + 16;
 constexpr int FV_PT = FV_S * BK / 256;   // weights per thread and slice
 
-__global__ __launch_bounds__(256) void k_fav_mfma(const double *__restrict__ YT,
+__global__ __launch_bounds__(256, 2) void k_fav_mfma(const double *__restrict__ YT,
                                                   const double *__restrict__ XtT,
                                                   const double *__restrict__ W,
                                                   double *__restrict__ Vp, int Np, int npd,
@@ -458,7 +458,7 @@ __device__ inline void adj_load(const AdjArgs &g, int step, int i0, int c0, doub
   }
 }
 
-__global__ __launch_bounds__(256) void k_adj_grad(AdjArgs g) {
+__global__ __launch_bounds__(256, 2) void k_adj_grad(AdjArgs g) {
   __shared__ double As[2][BK * LDS_LD];
   __shared__ double Bs[2][BK * LDS_LD];
   const int i0 = blockIdx.x * BM, c0 = blockIdx.y * BN;
