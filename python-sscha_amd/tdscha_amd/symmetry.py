@@ -34,7 +34,10 @@ def atom_permutations(positions, cell, rotations, translations, species=None, to
 
     positions (nat, 3) and translations (n_sym, 3) Cartesian (any length unit, the same as
     cell, whose rows are the supercell vectors); rotations (n_sym, 3, 3) Cartesian. Raises if
-    an image lands on no atom (or on one of another species) within `tol` (fractional)."""
+    an image lands on no atom (or on one of another species) within `tol` (fractional).
+    Images are matched by their fractional coordinates rounded to a grid of spacing ~tol/2
+    (sorted keys, one search per image); an image next to a grid edge falls back to the
+    nearest atom."""
     pos = np.asarray(positions, dtype=np.float64)
     cell = np.asarray(cell, dtype=np.float64)
     inv = np.linalg.inv(cell)
@@ -46,30 +49,31 @@ def atom_permutations(positions, cell, rotations, translations, species=None, to
     def key(f):
         f = f - np.floor(f)
         k = np.round(f * grid).astype(np.int64) % grid
-        return k[..., 0] * grid * grid + k[..., 1] * grid + k[..., 2]
+        return (k[..., 0] * grid + k[..., 1]) * grid + k[..., 2]
 
-    table = {}
-    for a, kk in enumerate(key(frac)):
-        table[int(kk)] = a
+    ref = key(frac)
+    order = np.argsort(ref)
+    ref_sorted = ref[order]
     rot = np.asarray(rotations, dtype=np.float64)
     tr = np.asarray(translations, dtype=np.float64)
     irt = np.empty((len(rot), nat), dtype=np.intc)
     for s in range(len(rot)):
         img = (pos @ rot[s].T + tr[s]) @ inv
-        for a, kk in enumerate(key(img)):
-            b = table.get(int(kk))
-            d = None if b is None else img[a] - frac[b]
-            if d is None or np.abs(d - np.round(d)).max() > tol:
-                # near a rounding edge of the lookup grid: nearest atom modulo the cell
-                dd = img[a][None] - frac
-                dd = np.abs(dd - np.round(dd)).max(axis=1)
-                b = int(np.argmin(dd))
-                if dd[b] > tol:
-                    raise ValueError("symmetry %d sends atom %d to no atom" % (s, a))
-            if spec[b] != spec[a]:
-                raise ValueError("symmetry %d sends atom %d to an atom of another species"
-                                 % (s, a))
-            irt[s, a] = b
+        k = key(img)
+        j = np.minimum(np.searchsorted(ref_sorted, k), nat - 1)
+        b = order[j]
+        d = img - frac[b]
+        bad = (ref_sorted[j] != k) | (np.abs(d - np.round(d)).max(axis=1) > tol)
+        for a in np.nonzero(bad)[0]:
+            dd = img[a][None] - frac
+            dd = np.abs(dd - np.round(dd)).max(axis=1)
+            b[a] = int(np.argmin(dd))
+            if dd[b[a]] > tol:
+                raise ValueError("symmetry %d sends atom %d to no atom" % (s, a))
+        if np.any(spec[b] != spec):
+            a = int(np.nonzero(spec[b] != spec)[0][0])
+            raise ValueError("symmetry %d sends atom %d to an atom of another species" % (s, a))
+        irt[s] = b
     return irt
 
 

@@ -118,3 +118,17 @@ def test_degenerate_blocks_and_no_sym():
     assert np.array_equal(lz.sym_block_id, np.arange(6))
     with pytest.raises(ValueError):
         lz.prepare_symmetrization()
+
+
+def test_atom_permutations_match_the_synthetic_crystal():
+    """Position matching agrees with the synthetic generator's integer-lattice permutations
+    (rock salt 3x3x3, 48 x 27 operations)."""
+    from tdscha_amd import symmetry as sy
+    from tdscha_amd import synthetic as syn
+    cr = syn.make_crystal(3)
+    rots, perms = syn.space_group(cr)
+    trans = cr.cell[cr.species == 0] @ syn.PRIM
+    T = np.array([t for _ in syn.oh_operations() for t in trans], dtype=float)
+    irt = sy.atom_permutations(cr.pos.astype(float), 3 * syn.PRIM.astype(float), rots, T,
+                               cr.species)
+    assert np.array_equal(irt, perms)
