@@ -174,8 +174,9 @@ int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, 
  * double-buffered LDS), 3 (256/192 x 192, A operands in registers, only B through LDS),
  * 4 (the same with 96-column tiles, two workgroups per CU) or 5 (GEMM1 of 4, GEMM2 of 3);
  * pass_slots image slots per pass (0 = all s_batch).
- * Results do not depend on pass_slots; the tile shape only changes the order of the FP64
- * row sums of GEMM1. */
+ * Results do not depend on pass_slots. Forms 0-2 differ from each other only in the order of
+ * GEMM1's FP64 row sums, as do forms 3-5; forms 3-5 add each lane's row-sum products in FP32
+ * and form gamma / delta in an FP64 product (agreement with forms 0-2: ~1e-7 relative). */
 int tdscha_ensemble_set_tuning(tdscha_ensemble *ens, int bf16_tiles, int pass_slots);
 int tdscha_ensemble_get_tuning(const tdscha_ensemble *ens, int *bf16_tiles, int *pass_slots);
 

@@ -219,17 +219,21 @@ __global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm1_bf16x3(Gemm1Args g) 
 // W[s] = (c1, c2, e1, e2) from the column-tile row sums, tiles added in increasing order.
 // Wf (when not null) receives c1 and c2 / 2 in FP32, the weights as the GEMM2 operand
 // formation uses them.
-__global__ void k_weights_bf16(const double *__restrict__ P, const double *__restrict__ rho,
-                               double *__restrict__ W, float *__restrict__ Wf, int Np, int n_ct,
-                               int sb, double kf, double kr, double ky) {
+__global__ void k_weights_bf16(const double *__restrict__ P, const double *__restrict__ GD,
+                               const double *__restrict__ rho, double *__restrict__ W,
+                               float *__restrict__ Wf, int Np, int n_ct, int nq, int sb,
+                               int gd_stride, double kf, double kr, double ky) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = blockIdx.y;
   if (i >= Np) return;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-  const double *Ps = P + (size_t)sl * n_ct * 4 * Np + i;
+  const double *Ps = P + (size_t)sl * n_ct * nq * Np + i;
   for (int ct = 0; ct < n_ct; ++ct)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) s[q] += Ps[((size_t)ct * 4 + q) * Np];
+    for (int q = 0; q < nq; ++q) s[q] += Ps[((size_t)ct * nq + q) * Np];
+  if (nq == 2 && GD) {
+    s[2] = GD[(size_t)sl * Np + i];
+    s[3] = GD[((size_t)gd_stride + sl) * Np + i];
+  }
   const double r = rho[i];
   double *Ws = W + (size_t)sl * 4 * Np;
   const double c1 = r * (ky * s[0] / 8 - kr * s[2] / 3);
@@ -615,7 +619,7 @@ void gemm1_launch_dma(const Ensemble &e, int sb, int what, hipStream_t st) {
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL(k_gemm1_bf16x3_dma, dim3(8 * ((total + 7) / 8)), dim3(DNT), 0, st, g);
-  launch_weights_bf16(e, sb, g.n_ct, what, st);
+  launch_weights_bf16(e, sb, g.n_ct, 4, what, st);
 }
 
 void gemm2_launch_dma(const Ensemble &e, int sb, hipStream_t st) {
@@ -641,7 +645,7 @@ void gemm1_launch(const Ensemble &e, int sb, int what, hipStream_t st) {
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL((k_gemm1_bf16x3<TM, TN>), dim3(8 * ((total + 7) / 8)), dim3(2 * TN), 0, st,
                      g);
-  launch_weights_bf16(e, sb, g.n_ct, what, st);
+  launch_weights_bf16(e, sb, g.n_ct, 4, what, st);
 }
 
 template <int TM, int TN>
@@ -660,12 +664,13 @@ void gemm2_launch(const Ensemble &e, int sb, hipStream_t st) {
 
 int bf16_col_tiles(const Ensemble &e) { return (e.npd + kBfTileN - 1) / kBfTileN; }
 
-void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStream_t st) {
+void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, hipStream_t st) {
   const double kf = (what & kTermF) ? 1.0 : 0.0;
   const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
   const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
-  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw, e.rho,
-                     e.W, e.Wf, e.Np, n_ct, sb, kf, kr, ky);
+  hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw,
+                     kr != 0.0 ? e.GD : nullptr, e.rho, e.W, e.Wf, e.Np, n_ct, nq, sb, e.s_batch,
+                     kf, kr, ky);
 }
 
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {

@@ -203,41 +203,37 @@ struct Gemm1RArgs {
   const uint16_t *Xh, *Xl;    // [Np x npd] fragment-tiled
   const uint16_t *Yh, *Yl;    // [s_batch x npd x npd] planes of the rotated Y1
   const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
-  const double *G;            // [s_batch x 2 x npd]
-  double *P;                  // [sb x n_ct x 4 x Np] row sums per column tile
+  double *P;                  // [sb x n_ct x 2 x Np] row sums per column tile
   int Np, npd, sb, n_it, n_ct, do_q;
 };
 
-// FP64 row sums of block row ti of this wave over its columns below nj (alpha, beta, gamma,
-// delta; the first two from Q), reduced over the 16 lanes of each row group with DPP.
+// Row sums of block row ti of this wave over its columns below nj: alpha = sum X Q and
+// beta = sum Ft Q, each lane's six products added in FP32 (Q itself carries the BF16x3
+// error), then reduced over the 16 lanes of each row group in FP64 with DPP. gamma and delta
+// do not involve Q and come from an FP64 product outside this kernel.
 template <int MB>
-__device__ inline void row_sums(const Gemm1RArgs &g, int sl, int ti, int row0, int cw, int nj,
-                                const v4f (&acc)[MB][RNB], double (&p)[4][4]) {
+__device__ inline void row_sums(const Gemm1RArgs &g, int ti, int row0, int cw, int nj,
+                                const v4f (&acc)[MB][RNB], double (&p)[4][2]) {
   const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
-  const double *g1 = g.G + (size_t)sl * 2 * g.npd, *g2 = g1 + g.npd;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) p[r][0] = p[r][1] = p[r][2] = p[r][3] = 0.0;
+  float pa[4] = {0.f, 0.f, 0.f, 0.f}, pb[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int tj = 0; tj < RNB; ++tj) {
     if (tj < nj) {
       const int c = cw + tj * 16 + lr;
-      const double gg1 = g1[c], gg2 = g2[c];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const size_t o = (size_t)(row0 + ti * 16 + 4 * lg + r) * g.npd + c;
-        const double x = g.Xf[o], ft = g.Ftf[o];
-        const double q = (double)acc[ti][tj][r];
-        p[r][0] = fma(x, q, p[r][0]);
-        p[r][1] = fma(ft, q, p[r][1]);
-        p[r][2] = fma(x, gg1, p[r][2]);
-        p[r][3] = fma(ft, gg2, p[r][3]);
+        const float q = acc[ti][tj][r];
+        pa[r] = fmaf(g.Xf[o], q, pa[r]);
+        pb[r] = fmaf(g.Ftf[o], q, pb[r]);
       }
     }
   }
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) p[r][q] = row16_sum(p[r][q]);
+  for (int r = 0; r < 4; ++r) {
+    p[r][0] = row16_sum((double)pa[r]);
+    p[r][1] = row16_sum((double)pb[r]);
+  }
 }
 
 template <int WCN>
@@ -296,38 +292,38 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
     }
   }
 
-  // epilogue: FP64 row sums over this wave's 96 columns, one 16-row block at a time (a
-  // 16-lane DPP reduction adds the 16 columns of a block). With two column waves, both leave
-  // their sums in LDS (the stages are free: the loop ended on a barrier) and the workgroup
-  // adds the two halves, wave 0's first, in coalesced stores of the tile's row sums.
-  double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 4 * g.Np;
+  // epilogue: row sums over this wave's 96 columns, one 16-row block at a time (a 16-lane
+  // DPP reduction adds the 16 columns of a block). With two column waves, both leave their
+  // sums in LDS (the stages are free: the loop ended on a barrier) and the workgroup adds the
+  // two halves, wave 0's first, in coalesced stores of the tile's row sums.
+  double *Pt = g.P + ((size_t)sl * g.n_ct + ct) * 2 * g.Np;
   if constexpr (WCN == 1) {
 #pragma unroll
     for (int ti = 0; ti < MB; ++ti) {
-      double p[4][4];
-      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
+      double p[4][2];
+      row_sums<MB>(g, ti, row0, cw, nj, acc, p);
       if (lr == 0)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < 2; ++q)
             Pt[(size_t)q * g.Np + row0 + ti * 16 + 4 * lg + r] = p[r][q];
     }
   } else {
-    double(*red)[TM][4] = reinterpret_cast<double(*)[TM][4]>(&st[0]);   // [2][256][4]: 16 KB
-    static_assert(sizeof(double) * 2 * TM * 4 <= sizeof(st), "row-sum halves exceed the stages");
+    double(*red)[TM][2] = reinterpret_cast<double(*)[TM][2]>(&st[0]);   // [2][256][2]: 8 KB
+    static_assert(sizeof(double) * 2 * TM * 2 <= sizeof(st), "row-sum halves exceed the stages");
 #pragma unroll
     for (int ti = 0; ti < MB; ++ti) {
-      double p[4][4];
-      row_sums<MB>(g, sl, ti, row0, cw, nj, acc, p);
+      double p[4][2];
+      row_sums<MB>(g, ti, row0, cw, nj, acc, p);
       if (lr == 0)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) red[wc][wr * 64 + ti * 16 + 4 * lg + r][q] = p[r][q];
+          for (int q = 0; q < 2; ++q) red[wc][wr * 64 + ti * 16 + 4 * lg + r][q] = p[r][q];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 4 * TM; t += RCfg<WCN>::NT) {
+    for (int t = threadIdx.x; t < 2 * TM; t += RCfg<WCN>::NT) {
       const int q = t / TM, m = t - q * TM;
       Pt[(size_t)q * g.Np + i0 + m] = red[0][m][q] + red[1][m][q];
     }
@@ -468,18 +464,26 @@ void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStr
   const size_t NX = (size_t)e.Np * e.npd;
   g.Xh = e.X_frag; g.Xl = e.X_frag + NX;
   g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
-  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
+  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   const int tn = RWC * wcn;
   g.n_it = e.Np / 256; g.n_ct = (e.npd + tn - 1) / tn;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  if (what & kTermD2vR) {
+    // gamma_i = X_i . (f_ups r_s) = sum_c r_s[c] X~^T[c][i] and delta_i = Ft_i . (r_s / f_psi)
+    // = sum_c r_s[c] Y^T[c][i], in FP64 on the FP64 operand copies
+    launch_weighted_tn(e.npd, sb, e.Np, e.Rv, e.s_batch, nullptr, e.XtT, e.Np, 1.0, e.GD, e.Np,
+                       st);
+    launch_weighted_tn(e.npd, sb, e.Np, e.Rv, e.s_batch, nullptr, e.YT, e.Np, 1.0,
+                       e.GD + (size_t)e.s_batch * e.Np, e.Np, st);
+  }
   const int total = sb * g.n_it * g.n_ct;
   const dim3 grid(8 * ((total + 7) / 8));
   if (wcn == 2)
     hipLaunchKernelGGL(k_gemm1_bf16x3_rp<2>, grid, dim3(RCfg<2>::NT), 0, st, g);
   else
     hipLaunchKernelGGL(k_gemm1_bf16x3_rp<1>, grid, dim3(RCfg<1>::NT), 0, st, g);
-  launch_weights_bf16(e, sb, g.n_ct, what, st);
+  launch_weights_bf16(e, sb, g.n_ct, 2, what, st);
 }
 
 void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st) {

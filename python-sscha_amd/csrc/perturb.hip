@@ -107,6 +107,15 @@ __global__ __launch_bounds__(256, 2) void k_weighted_tn(WtnArgs g) {
 }
 
 }  // namespace
+
+void launch_weighted_tn(int K, int M, int N, const double *A, int lda, const double *w,
+                        const double *B, int ldb, double scale, double *C, int ldc,
+                        hipStream_t st) {
+  WtnArgs g{A, w, B, C, K, M, N, lda, ldb, ldc, scale};
+  hipLaunchKernelGGL(k_weighted_tn, dim3((N + WN - 1) / WN, (M + WM - 1) / WM), dim3(256), 0, st,
+                     g);
+}
+
 }  // namespace tdscha
 
 using namespace tdscha;
@@ -131,9 +140,7 @@ extern "C" int tdscha_weighted_tn_product(int K, int M, int N, const double *A, 
     set_error("arrays too large");
     return 2;
   }
-  WtnArgs g{A, w, B, C, K, M, N, lda, ldb, ldc, scale};
-  hipLaunchKernelGGL(k_weighted_tn, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0,
-                     (hipStream_t)stream, g);
+  launch_weighted_tn(K, M, N, A, lda, w, B, ldb, scale, C, ldc, (hipStream_t)stream);
   const hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     set_error(std::string("launch failed: ") + hipGetErrorString(err));

@@ -376,33 +376,30 @@ __global__ __launch_bounds__(256) void k_rot2_grp(const TB *__restrict__ B,
   }
 }
 
-// r_s = S_s^T R1;  G[s][0][a] = f_ups[a] r_s[a],  G[s][1][a] = r_s[a] / f_psi[a]
+// r_s = S_s^T R1;  G[s][0][a] = f_ups[a] r_s[a],  G[s][1][a] = r_s[a] / f_psi[a];
+// Rv[a][s] = r_s[a] when Rv is given (leading dimension ldr)
 __global__ void k_rot_vec(const double *__restrict__ R1p, double *__restrict__ G,
                           const double *__restrict__ S, const ModeMeta *__restrict__ meta,
                           const double *__restrict__ fups, const double *__restrict__ fpsi, int n,
-                          int npd, int sym_stride, Pass ps) {
+                          int npd, int sym_stride, Pass ps, double *__restrict__ Rv, int ldr) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   int sl = blockIdx.y;
   if (a >= npd) return;
-  double g1 = 0.0, g2 = 0.0;
+  double g1 = 0.0, g2 = 0.0, r = 0.0;
   R1p += (size_t)ps.chain(sl) * npd;
   if (a < n) {
     ModeMeta ma = meta[a];
     const int ap = a - ma.blk_off, d = ma.blk_dim;
     const double *Sa = S + (size_t)ps.sym(sl) * sym_stride + ma.sym_off;
-    double r = 0.0;
     for (int k = 0; k < d; ++k) r = fma(Sa[k * d + ap], R1p[ma.blk_off + k], r);
     g1 = fups[a] * r;
     g2 = r / fpsi[a];
   }
   G[(size_t)sl * 2 * npd + a] = g1;
   G[(size_t)sl * 2 * npd + npd + a] = g2;
+  if (Rv) Rv[(size_t)a * ldr + sl] = r;
 }
 
-// f_acc[chain][a] += sum over the chain's slots of sum_k S_s[a'][k] V[slot][off_A + k]
-// f_acc[c][a] += sum over the chain's slots t (in order) of (S_s^T V_s)[a]: the products are
-// formed for all (a, slot) at once into P (one thread each), then summed per chain in slot
-// order, so the result is the one of a sequential loop over the slots.
 __global__ void k_fav_rot_part(const double *__restrict__ V, double *__restrict__ P,
                                const double *__restrict__ S, const ModeMeta *__restrict__ meta,
                                int n, int npd, int sym_stride, Pass ps) {
@@ -559,14 +556,16 @@ void launch_rot2_fused(const Ensemble &e, const Pass &ps, bool b_f32, hipStream_
 }
 
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st) {
-  launch_rot_vec_from(e, ps, e.R1p, e.G, e.fpsi, st);
+  dim3 grid((e.npd + 255) / 256, ps.slots());
+  hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, e.R1p, e.G, e.S, e.meta, e.fups, e.fpsi,
+                     e.n, e.npd, e.sym_stride, ps, e.Rv, e.s_batch);
 }
 
 void launch_rot_vec_from(const Ensemble &e, const Pass &ps, const double *in, double *out,
                          const double *div, hipStream_t st) {
   dim3 grid((e.npd + 255) / 256, ps.slots());
   hipLaunchKernelGGL(k_rot_vec, grid, dim3(256), 0, st, in, out, e.S, e.meta, e.fups, div, e.n,
-                     e.npd, e.sym_stride, ps);
+                     e.npd, e.sym_stride, ps, (double *)nullptr, 0);
 }
 
 void launch_fav_rot_acc(const Ensemble &e, const Pass &ps, hipStream_t st) {

@@ -241,10 +241,14 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   rc |= dev_alloc(h, &e.V, sb * e.npd);
   rc |= dev_alloc(h, &e.Vp, (size_t)((e.Np + kFavChunk - 1) / kFavChunk) * sb * e.npd);
   e.Pw = nullptr;
+  e.Rv = nullptr;
+  e.GD = nullptr;
   e.Wf = nullptr;
   e.B_f = nullptr;
   if (e.prec_mask & (1 << kBF16x3)) {
     rc |= dev_alloc(h, &e.Pw, sb * bf16_col_tiles(e) * 4 * (size_t)e.Np);
+    rc |= dev_alloc(h, &e.Rv, sb * (size_t)e.npd);
+    rc |= dev_alloc(h, &e.GD, sb * 2 * (size_t)e.Np);
     rc |= dev_alloc(h, &e.Wf, sb * 2 * (size_t)e.Np);
     rc |= dev_alloc(h, &e.B_f, sb * nn);
   }

@@ -114,6 +114,8 @@ struct Ensemble {
   double *V = nullptr;        // [s_batch x npd]  Y^T e1 + X~^T e2
   double *Vp = nullptr;       // [ceil(Np / kFavChunk) x s_batch x npd]  partial sums of V
   double *Pw = nullptr;       // [s_batch x col tiles x 4 x Np]  BF16x3 GEMM1 row sums
+  double *Rv = nullptr;       // [npd x s_batch]  r_s = S_s^T R1 of every slot (BF16x3)
+  double *GD = nullptr;       // [2 x s_batch x Np]  gamma, delta of every image (BF16x3)
   // force adjoint (allocated by its first call): per-chain seeds, per-slot rotated seeds,
   // their row sums and the gradient accumulator
   int adj_chains = 0, adj_slots = 0;
@@ -142,6 +144,10 @@ void launch_rot_left(const Ensemble &e, const double *in, double *out, const Pas
                      bool accumulate, hipStream_t st);
 void launch_rot_vec(const Ensemble &e, const Pass &ps, hipStream_t st);
 // out[slot] = (f_ups * S^T in[chain], S^T in[chain] / div)
+// C[m][n] = scale sum_k A[k][m] w[k] B[k][n] (perturb.hip; w may be null)
+void launch_weighted_tn(int K, int M, int N, const double *A, int lda, const double *w,
+                        const double *B, int ldb, double scale, double *C, int ldc,
+                        hipStream_t st);
 void launch_rot_vec_from(const Ensemble &e, const Pass &ps, const double *in, double *out,
                          const double *div, hipStream_t st);
 // out[slot] = S^T in[chain] S (FP64), optionally scaled out[a][b] *= scale[a] scale[b]
@@ -181,7 +187,9 @@ int bf16_col_tiles(const Ensemble &e);
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st);
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st);
 // per-image weights W (and their FP32 copy Wf) from GEMM1's row sums over n_ct column tiles
-void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int what, hipStream_t st);
+// nq = 4: P holds alpha, beta, gamma, delta per column tile; nq = 2: alpha, beta, with gamma
+// and delta in e.GD
+void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, hipStream_t st);
 // wave-private-operand forms (gemm_bf16_rp.hip); GEMM2 writes B_s^T
 void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st);
 // fp64 [rows x cols] -> fragment-tiled bf16 hi / lo planes (dst, dst + rows * cols)

@@ -146,19 +146,27 @@ def test_group_of_uneven_systems_matches_members_alone():
 
 
 def test_autotuned_pass_size_keeps_results():
-    """The autotuner only changes how the images are split into passes; the per-image
-    contributions are added in the same order, so L is bit-identical."""
+    """The pass size the autotuner picks only changes how the images are split into passes;
+    the per-image contributions are added in the same order, so L is bit-identical for the
+    same tile form. The tile form it picks agrees with the initial one to the FP32 rounding
+    of the row-sum partials (1e-6 relative)."""
     import torch
     ens = fixtures.random_ensemble(65, 60, 90, 24)
     dev = _dev_ens(ens, precisions=("fp64", "bf16x3"), s_batch=24)
     psi = torch.as_tensor(np.random.default_rng(66).standard_normal(orc.psi_len(60)),
                           device=dev.device)
+    form0 = dev.tuning()[0]
     before = [dev.apply_full_L(psi, precision=p).clone() for p in ("fp64", "bf16x3")]
     slots, per = dev.autotune("bf16x3", 2)
     assert 2 <= slots <= 24 and per > 0
-    after = [dev.apply_full_L(psi, precision=p) for p in ("fp64", "bf16x3")]
+    form = dev.tuning()[0]
+    dev.set_tuning(form0, slots)
+    after = [dev.apply_full_L(psi, precision=p).clone() for p in ("fp64", "bf16x3")]
     for x, y in zip(before, after):
         assert torch.equal(x, y)
+    dev.set_tuning(form, slots)
+    tuned = dev.apply_full_L(psi, precision="bf16x3")
+    assert torch.max(torch.abs(tuned - before[1])) <= 1e-6 * torch.max(torch.abs(before[1]))
 
 
 def test_batched_driver_small():

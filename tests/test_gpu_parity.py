@@ -97,8 +97,11 @@ def test_group_tiled_rotation_widths(precision, max_block, seed):
 
 @pytest.mark.parametrize("n,N,seed", [(150, 300, 31), (300, 520, 32), (45, 256, 33)])
 def test_bf16x3_tile_shapes(n, N, seed):
-    """The six contraction kernel forms against the oracle, and against each other (they
-    differ only in the order of GEMM1's FP64 row sums)."""
+    """The six contraction kernel forms against the oracle and against each other. Forms 0-2
+    share their arithmetic and differ only in the order of GEMM1's FP64 row sums (1e-12);
+    so do forms 3-5 among themselves. Forms 3-5 add each lane's six alpha / beta products in
+    FP32 and take gamma / delta from an FP64 product, so they agree with forms 0-2 to the
+    FP32 rounding of those partial sums (1e-6 relative)."""
     ens = fixtures.random_ensemble(seed, n, N, 4, max_block=8)
     R1, Y1 = fixtures.random_perturbation(seed + 1, n)
     dev = _dev_ens(ens, precisions=("fp64", "bf16x3"))
@@ -107,9 +110,10 @@ def test_bf16x3_tile_shapes(n, N, seed):
         dev.set_tuning(v, 0)
         assert dev.tuning()[0] == v
         out.append(_check(ens, R1, Y1, 1, dev=dev, tol=2e-4, precision="bf16x3"))
-    for o in out[1:]:
-        for a, b in zip(out[0], o):
-            assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(a))
+    for ref, group, tol in ((0, (1, 2), 1e-12), (3, (4, 5), 1e-12), (0, (3,), 1e-6)):
+        for k in group:
+            for a, b in zip(out[ref], out[k]):
+                assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(a)), (ref, k)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("bf16x3", 2e-4)])
