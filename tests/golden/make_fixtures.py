@@ -26,6 +26,8 @@ Outputs (tests/golden/):
                         file that prepare_input_files (Py:862-919) wrote
   raman_unpol.npz       tests/test_raman_unpol: the Raman tensor and the seven unpolarized
                         Raman vectors its test checks (raman_tensor_eq.npy, raman_v_*.npy)
+  lanczos_status_step5.npz  tests/test_lanczos_fast/normal/LANCZOS_STEP5.npz as the
+                        reference's save_status wrote it (copied byte for byte)
 """
 import os
 import sys
@@ -144,6 +146,9 @@ def main():
     projection_fixture()
     input_files_fixture()
     raman_fixture()
+    import shutil
+    shutil.copyfile(os.path.join(REF, "tests/test_lanczos_fast/normal/LANCZOS_STEP5.npz"),
+                    os.path.join(HERE, "lanczos_status_step5.npz"))
     for f in sorted(os.listdir(HERE)):
         print(f, os.path.getsize(os.path.join(HERE, f)))
 

@@ -175,3 +175,25 @@ def test_project_entry_point_rejects_bad_arguments():
     assert lib.tdscha_project_ensemble(1, 3, 1, None, p, None, None, p, p, BOHR, p, p, None) == 1
     assert lib.tdscha_project_ensemble(1, 3, 1, p, p, None, None, p, p, 0.0, p, p, None) == 2
     assert b"bohr" in lib.tdscha_last_error()
+
+
+def test_load_status_reads_a_reference_status_file(mode10):
+    """The npz the reference's save_status wrote (its 5-step run on the SnTe ensemble) loads
+    without pickle: ensemble, padded symmetry blocks (block sizes recovered from the padding),
+    Krylov coefficients and bases."""
+    from tdscha_amd import DynamicalLanczos as DL
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64)
+    lz.load_status(os.path.join(GOLDEN, "lanczos_status_step5.npz"))
+    assert np.array_equal(lz.X, mode10.X) and np.array_equal(lz.Y, mode10.Y)
+    assert np.array_equal(lz.w, mode10.w) and lz.T == mode10.T
+    assert lz.n_syms == 384 and len(lz.symmetries) == 9
+    s = mode10.sym
+    assert np.array_equal(lz.N_degeneracy, s.N_degeneracy)
+    assert np.array_equal(lz.sym_block_id, s.block_id)
+    for a, b in zip(lz.symmetries, s.symmetries):
+        assert np.array_equal(a, b)
+    std = load("lanczos_std.npz")        # the 10-step run of the same calculation
+    assert len(lz.a_coeffs) == 5
+    assert np.allclose(lz.a_coeffs, std["a"][:5], rtol=1e-12, atol=0)
+    assert np.allclose(lz.b_coeffs, std["b"][:5], rtol=1e-12, atol=0)
+    assert np.array_equal(np.asarray(lz.basis_Q)[:6], std["Q"][:6])
