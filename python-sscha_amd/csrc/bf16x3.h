@@ -34,6 +34,25 @@ __device__ inline int xcd_tile(int total) {
   return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
 
+// Tile `id` of a contraction over sb image slots with n_it row tiles and n_ct column tiles.
+// Slots go in groups of sg (the last group may be shorter); inside a group the column tile
+// is slowest, then the row tile, and the slot fastest, so that the tiles an XCD runs together
+// share the row operand and any slot-independent epilogue operands and, per slot, the column
+// panel.
+struct SlotTile {
+  int sl, it, ct;
+};
+__device__ inline SlotTile slot_group_tile(int id, int sb, int sg, int n_it, int n_ct) {
+  const int per_g = sg * n_it * n_ct;
+  const int grp = id / per_g, s0 = grp * sg;
+  const int gsz = min(sg, sb - s0);
+  const int rest = id - grp * per_g;
+  const int ct = rest / (n_it * gsz);
+  const int r2 = rest - ct * n_it * gsz;
+  const int it = r2 / gsz;
+  return SlotTile{s0 + (r2 - it * gsz), it, ct};
+}
+
 // Number of 16-wide blocks, out of `cap` starting at `start`, below `valid`; wave-uniform.
 __device__ inline int clamp_blocks(int valid, int start, int cap) {
   int b = (valid - start + 15) / 16;

@@ -117,7 +117,7 @@ struct Gemm1Args {
   const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
   const double *G;            // [s_batch x 2 x npd]
   double *P;                  // [sb x n_ct x 4 x Np] row sums per column tile
-  int Np, npd, sb, n_it, n_ct, do_q;
+  int Np, npd, sb, n_it, n_ct, sg, do_q;   // sg: image slots per tile group
 };
 
 template <int TM, int TN>
@@ -128,10 +128,8 @@ __global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm1_bf16x3(Gemm1Args g) 
   const int total = g.sb * g.n_it * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
-  // slot-major, then row tile, column tile fastest
-  const int sl = id / (g.n_it * g.n_ct);
-  const int rest = id - sl * g.n_it * g.n_ct;
-  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  const SlotTile tt = slot_group_tile(id, g.sb, g.sg, g.n_it, g.n_ct);
+  const int sl = tt.sl, it = tt.it, ct = tt.ct;
   const int i0 = it * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = w / WC, wc = w % WC, lr = l & 15, lg = l >> 4;
@@ -444,9 +442,8 @@ __global__ __launch_bounds__(DNT, 1) void k_gemm1_bf16x3_dma(Gemm1Args g) {
   const int total = g.sb * g.n_it * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
-  const int sl = id / (g.n_it * g.n_ct);
-  const int rest = id - sl * g.n_it * g.n_ct;
-  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  const SlotTile tt = slot_group_tile(id, g.sb, g.sg, g.n_it, g.n_ct);
+  const int sl = tt.sl, it = tt.it, ct = tt.ct;
   const int i0 = it * DT, c0 = ct * DT;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = w >> 2, wc = w & 3, lr = l & 15, lg = l >> 4;
@@ -616,6 +613,7 @@ void gemm1_launch_dma(const Ensemble &e, int sb, int what, hipStream_t st) {
   g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   g.n_it = e.Np / DT; g.n_ct = (e.npd + DT - 1) / DT;
+  g.sg = sb < kG1SlotGroup ? sb : kG1SlotGroup;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL(k_gemm1_bf16x3_dma, dim3(8 * ((total + 7) / 8)), dim3(DNT), 0, st, g);
@@ -641,6 +639,7 @@ void gemm1_launch(const Ensemble &e, int sb, int what, hipStream_t st) {
   g.Xf = e.X_f; g.Ftf = e.Ft_f; g.G = e.G; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   g.n_it = e.Np / TM; g.n_ct = (e.npd + TN - 1) / TN;
+  g.sg = sb < kG1SlotGroup ? sb : kG1SlotGroup;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const int total = sb * g.n_it * g.n_ct;
   hipLaunchKernelGGL((k_gemm1_bf16x3<TM, TN>), dim3(8 * ((total + 7) / 8)), dim3(2 * TN), 0, st,

@@ -204,7 +204,7 @@ struct Gemm1RArgs {
   const uint16_t *Yh, *Yl;    // [s_batch x npd x npd] planes of the rotated Y1
   const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
   double *P;                  // [sb x n_ct x 2 x Np] row sums per column tile
-  int Np, npd, sb, n_it, n_ct, do_q;
+  int Np, npd, sb, n_it, n_ct, sg, do_q;   // sg: image slots per tile group
 };
 
 // Row sums of block row ti of this wave over its columns below nj: alpha = sum X Q and
@@ -243,10 +243,9 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
   const int total = g.sb * g.n_it * g.n_ct;
   const int id = xcd_tile(total);
   if (id >= total) return;
-  // slot-major, then row tile, column tile fastest
-  const int sl = id / (g.n_it * g.n_ct);
-  const int rest = id - sl * g.n_it * g.n_ct;
-  const int it = rest / g.n_ct, ct = rest - it * g.n_ct;
+  // the FP32 epilogue panel of a (row tile, column tile) is the same for every slot
+  const SlotTile tt = slot_group_tile(id, g.sb, g.sg, g.n_it, g.n_ct);
+  const int sl = tt.sl, it = tt.it, ct = tt.ct;
   const int i0 = it * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int wr = w & 3, wc = w >> 2;
@@ -468,6 +467,7 @@ void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStr
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   const int tn = RWC * wcn;
   g.n_it = e.Np / 256; g.n_ct = (e.npd + tn - 1) / tn;
+  g.sg = sb < kG1SlotGroup ? sb : kG1SlotGroup;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   if (what & kTermD2vR) {
     // gamma_i = X_i . (f_ups r_s) = sum_c r_s[c] X~^T[c][i] and delta_i = Ft_i . (r_s / f_psi)

@@ -29,6 +29,7 @@ constexpr double kRyToK = 157887.32400374097;  // LanczosFunctions.h:9
 constexpr int kRotTile = 64;   // widest block / column group of the fused rotations
 constexpr int kFavChunk = 1024; // configurations per split-k chunk of the f_av product
 constexpr int kBfTileN = 96;    // narrowest column tile of the BF16x3 GEMM1 row sums
+constexpr int kG1SlotGroup = 14; // image slots per GEMM1 tile group (slots run fastest in it)
 
 enum Precision { kFP64 = 0, kFP32 = 1, kBF16x3 = 2 };
 
