@@ -327,9 +327,11 @@ __global__ __launch_bounds__(256) void k_rot2_grp(const TB *__restrict__ B,
     const TB *Bs = B + (size_t)sl * npd * npd;
 #pragma unroll
     for (int q = 0; q < PT; ++q) {
+      // indices clamped to the groups (never empty): the padding re-reads lines already
+      // fetched instead of the neighbouring groups' elements
       const int idx = threadIdx.x + 256 * q, r = idx / G, c = idx - r * G;
-      bm[q] = Bs[(size_t)min(gg.x + r, npd - 1) * npd + min(gh.x + c, npd - 1)];
-      bt[q] = Bs[(size_t)min(gh.x + r, npd - 1) * npd + min(gg.x + c, npd - 1)];
+      bm[q] = Bs[(size_t)(gg.x + min(r, gg.y - 1)) * npd + gh.x + min(c, gh.y - 1)];
+      bt[q] = Bs[(size_t)(gh.x + min(r, gh.y - 1)) * npd + gg.x + min(c, gg.y - 1)];
     }
     group_s_load<G>(Ss, ig, sg);
     group_s_load<G>(Ss, ih, sh);
