@@ -26,6 +26,12 @@ Contents
   - symmetry operations on mode blocks      Py:735-746 (``symmetries_on_modes``)
   - ensemble averages of effective charges and Raman tensors, Modules/Perturbations.py
     :195-375 (``d1M_dR_av``, ``d1alpha_dR_av``, ``d2M_dR_av``, ``d2alpha_dR_av``)
+* Two-phonon (Wigner) perturbation sectors, Py:1488-1526 (``two_phonon_sectors``,
+  ``pack_two_phonon``), and the space-group symmetrization of the ensemble averages
+  (``symmetrize_average``, the role of Modules/Perturbations.py:15-193). The reference does the
+  latter with CellConstructor (absent here): its exact form (the acoustic sum rule in
+  particular) is parity unpinned; the restatement is pinned on invariance under every
+  operation, idempotence, the sum rule and group-invariant model tensors.
 * The derivative of a linear functional of the kernel outputs with respect to the
   configuration forces Y (``force_grad``), image by image in the reference's own form
   (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
@@ -736,3 +742,98 @@ def d2alpha_dR_av(u_bohr, rho, R, ups):
     """Perturbations.py:330-375 with u already in Bohr: (3, 3, 3 nat, 3 nat)."""
     v = np.einsum("ab, ib -> ia", ups, u_bohr)
     return np.einsum("i, ia, ibcd -> bcda", rho, v, R) / np.sum(rho)
+
+
+def two_phonon_sectors(d2_mu, w, T):
+    """Py:1484-1502: D = (d2 + d2^T) / 2, a'(1) = -sqrt(-chi^-/2) D, b'(1) = +sqrt(chi^+/2) D
+    (element by element)."""
+    D = 0.5 * (d2_mu + d2_mu.T)
+    chi_m, chi_p = chi_minus_plus(w, T)
+    pert_a = -np.einsum("nm, nm -> nm", np.sqrt(-0.5 * chi_m), D)
+    pert_b = +np.einsum("nm, nm -> nm", np.sqrt(+0.5 * chi_p), D)
+    return pert_a, pert_b
+
+
+def pack_two_phonon(psi, pert_a, pert_b):
+    """Py:1507-1516: rows i of the upper triangles (columns j >= i) one after the other, a'(1)
+    then b'(1), after the n one-phonon entries."""
+    n = pert_a.shape[0]
+    cur = n
+    for i in range(n):
+        psi[cur:cur + n - i] = pert_a[i, i:]
+        cur += n - i
+    for i in range(n):
+        psi[cur:cur + n - i] = pert_b[i, i:]
+        cur += n - i
+    return psi
+
+
+def _act(T, R, irt, n_atom):
+    """g.T for one operation, canonical layout (nat, 3) per atom axis then field axes of 3:
+    (g.T)[irt(a)][i] ... = R[i][p] ... T[a][p] ... (the convention of symmetries_on_modes)."""
+    nd = T.ndim
+    cart = [2 * k + 1 for k in range(n_atom)] + list(range(2 * n_atom, nd))
+    Tr = T
+    for ax in cart:                       # rotate every Cartesian axis
+        Tr = np.moveaxis(np.tensordot(R, Tr, axes=([1], [ax])), 0, ax)
+    out = np.empty_like(Tr)
+    if n_atom == 1:
+        out[irt] = Tr
+    else:
+        out[np.ix_(irt, np.arange(3), irt)] = Tr
+    return out
+
+
+def symmetrize_average(kind, C, rotations, irt, asr=True):
+    """The symmetrized ensemble averages of Modules/Perturbations.py:15-193, restated as the
+    space-group average (1/n_sym) sum_g g.T after the acoustic sum rule (the mean over the
+    atoms of every atom index removed) and, for the second-order tensors, the symmetrization
+    in the two displacements (and in the two field indices of the Raman tensors). Layouts as
+    d1M_dR_av ("d1M"), d1alpha_dR_av ("d1alpha"), d2M_dR_av ("d2M"), d2alpha_dR_av
+    ("d2alpha"). Loops over the operations: small systems only."""
+    C = np.asarray(C, dtype=np.float64)
+    rot = np.asarray(rotations, dtype=np.float64)
+    irt = np.asarray(irt)
+    if kind == "d1M":
+        nat = C.shape[0] // 3
+        T = C.reshape(nat, 3, 3)
+        if asr:
+            T = T - T.mean(axis=0, keepdims=True)
+        na = 1
+    elif kind == "d1alpha":
+        nat = C.shape[2] // 3
+        T = C.reshape(3, 3, nat, 3).transpose(2, 3, 0, 1)
+        T = 0.5 * (T + T.transpose(0, 1, 3, 2))
+        if asr:
+            T = T - T.mean(axis=0, keepdims=True)
+        na = 1
+    elif kind == "d2M":
+        nat = C.shape[0] // 3
+        T = C.reshape(nat, 3, nat, 3, 3)
+        T = 0.5 * (T + T.transpose(2, 3, 0, 1, 4))
+        if asr:
+            T = T - T.mean(axis=0, keepdims=True)
+            T = T - T.mean(axis=2, keepdims=True)
+        na = 2
+    elif kind == "d2alpha":
+        nat = C.shape[2] // 3
+        T = C.reshape(3, 3, nat, 3, nat, 3).transpose(2, 3, 4, 5, 0, 1)
+        T = 0.5 * (T + T.transpose(2, 3, 0, 1, 4, 5))
+        T = 0.5 * (T + T.transpose(0, 1, 2, 3, 5, 4))
+        if asr:
+            T = T - T.mean(axis=0, keepdims=True)
+            T = T - T.mean(axis=2, keepdims=True)
+        na = 2
+    else:
+        raise ValueError(kind)
+    S = np.zeros_like(T)
+    for g in range(len(rot)):
+        S += _act(T, rot[g], irt[g], na)
+    S /= len(rot)
+    if kind == "d1M":
+        return S.reshape(3 * nat, 3)
+    if kind == "d1alpha":
+        return S.transpose(2, 3, 0, 1).reshape(3, 3, 3 * nat)
+    if kind == "d2M":
+        return S.reshape(3 * nat, 3 * nat, 3)
+    return S.transpose(4, 5, 0, 1, 2, 3).reshape(3, 3, 3 * nat, 3 * nat)

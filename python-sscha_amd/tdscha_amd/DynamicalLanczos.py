@@ -100,6 +100,7 @@ class Lanczos(object):
         self.ignore_small_w = False
         self.effective_charges = None   # (nat_uc, 3, 3), read by prepare_ir
         self.raman_tensor = None        # (3, 3, 3 nat_uc), read by prepare_raman
+        self.sc_symmetries = None       # (rotations, irt) given to prepare_symmetrization
         self.L_linop = None
         self._dev_ens = None
         if ensemble is not None and hasattr(ensemble, "u_disps"):
@@ -189,6 +190,7 @@ class Lanczos(object):
         if pols.ndim != 2 or pols.shape[1] != n or pols.shape[0] % 3:
             raise ValueError("the polarization vectors of the kept modes are not set")
         rot, irt = symmetries
+        self.sc_symmetries = (np.asarray(rot, dtype=TYPE_DP), np.asarray(irt))
         t0 = time.time()
         starts = sy.degenerate_blocks(self.w, deg_tol)
         blocks = sy.symmetries_on_modes(rot, irt, pols, starts, device=self.device)
@@ -348,6 +350,232 @@ class Lanczos(object):
         if debug:
             np.save("raman_v_{}".format(index), v)
         self.prepare_perturbation(np.tile(v, self._n_supercell(T.shape[2])), masses_exp=-1)
+
+    # ------------------------------------------------------------------ anharmonic IR / Raman
+    def get_chi_minus(self):
+        """chi^- of the Wigner formalism (Py:2572-2598)."""
+        from . import perturbations as pt
+        return pt.chi_minus_plus(self.w, self.T, __EPSILON__)[0]
+
+    def get_chi_plus(self):
+        """chi^+ of the Wigner formalism (Py:2601-2627)."""
+        from . import perturbations as pt
+        return pt.chi_minus_plus(self.w, self.T, __EPSILON__)[1]
+
+    def _set_two_phonon(self, d2_mu, label):
+        """Fill psi's a'(1), b'(1) sectors from the mode-space matrix d2_mu and take the
+        perturbation modulus with the Wigner mask (Py:1488-1526)."""
+        from . import perturbations as pt
+        pert_a, pert_b = pt.wigner_two_phonon(d2_mu, self.w, self.T)
+        pt.pack_wigner_two_phonon(self.psi, pert_a, pert_b)
+        self.perturbation_modulus = self.psi.dot(self.psi * self.mask_dot_wigner())
+        if self.verbose:
+            print("Perturbation modulus after adding two ph contributions{} = {}".format(
+                label, self.perturbation_modulus))
+
+    def _to_mode_space(self, M):
+        """E^T diag(1/sqrt m) M diag(1/sqrt m) E for a (3 nat, 3 nat) matrix."""
+        sm = 1.0 / np.sqrt(self.m)
+        P = self.pols * sm[:, None]
+        return P.T @ M @ P
+
+    def _ensemble_arrays(self, ensemble, n_expected):
+        """u_disps (N, 3 nat) in Angstrom and rho of a duck-typed ensemble (attributes u_disps,
+        rho; the reference's sscha.Ensemble has both)."""
+        if ensemble is None or not hasattr(ensemble, "u_disps"):
+            raise ValueError("pass ensemble= (an object with u_disps (N, nat, 3) in Angstrom and "
+                             "rho (N)) for the ensemble averages")
+        u = np.asarray(ensemble.u_disps, dtype=TYPE_DP).reshape(len(ensemble.rho), -1)
+        rho = np.asarray(ensemble.rho, dtype=TYPE_DP)
+        if len(rho) != n_expected:
+            raise ValueError("the tensors are given for {} configurations, the ensemble has {}"
+                             .format(n_expected, len(rho)))
+        return u, rho
+
+    def _upsilon(self):
+        from . import perturbations as pt
+        return pt.upsilon_matrix(self.w, self.pols, self.m, self.T)
+
+    def _sym_ops(self, symmetrize):
+        if symmetrize and self.sc_symmetries is None:
+            raise ValueError("symmetrize=True uses the space group given to "
+                             "prepare_symmetrization(symmetries=(rotations, irt))")
+        return self.sc_symmetries if symmetrize else None
+
+    def prepare_anharmonic_ir_FT(self, ec=None, ec_eq=None, pol_vec_light=np.array([1., 0., 0.]),
+                                 add_two_ph=False, symmetrize=False, ensemble=None):
+        """IR perturbation with configuration-dependent effective charges (Py:1693-1824).
+
+        ec (N, nat_sc, 3 field, 3 Cartesian); ec_eq (nat_uc, 3, 3), subtracted (tiled over the
+        supercell) from ec in the two-phonon average only; ensemble: u_disps (Angstrom) and
+        rho of the configurations. One-phonon sector: <Z*> . pol / sqrt(m) on the modes;
+        two-phonon sectors (Wigner only): E^T m^-1/2 (Upsilon <u (x) Z*> . pol) m^-1/2 E,
+        symmetrized, weighted by sqrt(-chi^-/2), sqrt(chi^+/2). Upsilon is built from this
+        object's frequencies, polarization vectors, masses and temperature (the reference takes
+        the ensemble's current dynamical matrix, the same one in a converged run).
+        symmetrize: the averages are symmetrized over the space group given to
+        prepare_symmetrization (perturbations.symmetrize_average)."""
+        from . import perturbations as pt
+        if not self.use_wigner and add_two_ph:
+            raise NotImplementedError("The two phonon processes are implemented only in Wigner")
+        if ec is None:
+            raise ValueError("Must specify the effective charges for all configurations!")
+        ec = np.asarray(ec, dtype=TYPE_DP)
+        if ec.ndim != 4 or ec.shape[1] != self.nat or ec.shape[2:] != (3, 3):
+            raise ValueError("The effective charges in input have the wrong shape. The required "
+                             "is N_conf N_at_sc E_field cart")
+        u, rho = self._ensemble_arrays(ensemble, ec.shape[0])
+        ops = self._sym_ops(symmetrize)
+        pol = np.asarray(pol_vec_light, dtype=TYPE_DP)
+        d1 = pt.get_d1M_dR_av(rho, ec, self.device, symmetrize, ops)
+        self.prepare_perturbation(d1 @ pol, masses_exp=-1)
+        if self.verbose:
+            print("Pertubation modulus with one ph effects only = {}".format(
+                self.perturbation_modulus))
+        if not add_two_ph:
+            return
+        if ec_eq is not None:
+            ec_eq = np.asarray(ec_eq, dtype=TYPE_DP)
+            if ec_eq.ndim != 3 or ec_eq.shape[1:] != (3, 3):
+                raise ValueError("Error, effective charges of the wrong shape: {}".format(
+                    ec_eq.shape))
+            n_sc = self._n_supercell(3 * ec_eq.shape[0])
+            if not self.ignore_small_w and ec_eq.shape[0] * 3 * n_sc != self.n_modes + 3:
+                raise ValueError("effective charges do not match the modes")
+            ec = ec - np.tile(ec_eq, (n_sc, 1, 1))[None]
+        d2 = pt.get_d2M_dR_av(u, rho, ec, self._upsilon(), self.device,
+                              symmetrize=symmetrize, symmetries=ops)
+        self._set_two_phonon(self._to_mode_space(d2 @ pol), "")
+
+    def prepare_anharmonic_ir(self, ec=None, ec_eq=None, pol_vec_light=np.array([1., 0., 0.]),
+                              add_two_ph=False):
+        """IR perturbation with configuration-dependent effective charges, earlier form
+        (Py:1827-1960): the averages are taken on this object's ensemble in the mode basis,
+        <X_n f_ups(w_n) (E^T m^-1/2 Z* . pol)_m>, both as FP64 products on the device."""
+        import torch
+        from . import perturbations as pt
+        if not self.use_wigner and add_two_ph:
+            raise NotImplementedError("The two phonon processes are implemented only in Wigner")
+        if ec is None:
+            raise ValueError("Must specify the effective charges for all configurations!")
+        ec = np.asarray(ec, dtype=TYPE_DP)
+        if ec.shape != (self.N, self.nat, 3, 3):
+            raise ValueError("The effective charges in input have the wrong shape. The required "
+                             "is {}".format((self.N, self.nat, 3, 3)))
+        pol = np.asarray(pol_vec_light, dtype=TYPE_DP)
+        z = np.einsum("iabc, b -> iac", ec, pol).reshape(self.N, -1)     # (N, 3 nat)
+        d1 = self.rho @ z / np.sum(self.rho)
+        self.prepare_perturbation(d1, masses_exp=-1)
+        if self.verbose:
+            print("Pertubation modulus with one ph effects only = {}".format(
+                self.perturbation_modulus))
+        if not add_two_ph:
+            return
+        if ec_eq is not None:
+            ec_eq = np.asarray(ec_eq, dtype=TYPE_DP)
+            if ec_eq.ndim != 3 or ec_eq.shape[1:] != (3, 3):
+                raise ValueError("Error, effective charges of the wrong shape: {}".format(
+                    ec_eq.shape))
+            n_sc = self._n_supercell(3 * ec_eq.shape[0])
+            if not self.ignore_small_w and ec_eq.shape[0] * 3 * n_sc != self.n_modes + 3:
+                raise ValueError("effective charges do not match the modes")
+            z = z - np.tile(np.einsum("abc, b -> ac", ec_eq, pol).ravel(), n_sc)[None]
+        dev = pt._dev(self.device)
+        P = torch.as_tensor(self.pols / np.sqrt(self.m)[:, None], device=dev)    # (3 nat, n)
+        zt = torch.as_tensor(z, device=dev)
+        r = torch.as_tensor(self.rho, device=dev)
+        X = torch.as_tensor(np.ascontiguousarray(self.X), device=dev)
+        C1 = pt._tn(X, r, zt, 1.0 / float(self.rho.sum()), dev)              # <X (x) z>
+        d2 = pt._tn(C1.t().contiguous(), None, P, 1.0, dev).cpu().numpy()    # (n, n)
+        d2 = f_ups(self.w, self.T)[:, None] * d2
+        self._set_two_phonon(d2, "")
+
+    def prepare_anharmonic_raman_FT(self, raman=None, raman_eq=None,
+                                    pol_in=np.array([1., 0., 0.]), pol_out=np.array([1., 0., 0.]),
+                                    mixed=False, pol_in_2=None, pol_out_2=None, pol_in_3=None,
+                                    pol_out_3=None, add_two_ph=False, symmetrize=False,
+                                    ensemble=None, save_raman_tensor2=False,
+                                    file_raman_tensor2=None):
+        """Raman perturbation with configuration-dependent Raman tensors (Py:1340-1532).
+
+        raman (N, 3, 3, 3 nat_sc); raman_eq (3, 3, 3 nat_uc), subtracted (tiled) in the
+        two-phonon average only. One-phonon sector: the Raman vector of <dalpha/dR> for
+        (pol_in, pol_out) (plus the _2 and _3 pairs when mixed), with the acoustic sum rule of
+        raman_vector; two-phonon sectors as prepare_anharmonic_ir_FT with the second-order
+        Raman tensor. save_raman_tensor2: np.save the mass-scaled second-order tensor to
+        file_raman_tensor2 and return before the two-phonon sectors, as the reference does."""
+        from . import perturbations as pt
+        if not self.use_wigner and add_two_ph:
+            raise NotImplementedError("The two phonon processes are implemented only in Wigner")
+        if raman is None:
+            raise ValueError("Must specify the raman tensors for all configurations!")
+        pairs = [(pol_in, pol_out)]
+        if mixed:
+            for a, b, k in ((pol_in_2, pol_out_2, 2), (pol_in_3, pol_out_3, 3)):
+                if a is None or b is None:
+                    raise ValueError("Must specify pol_in_{0} pol_out_{0} if mixed = True!"
+                                     .format(k))
+                if len(a) != 3 or len(b) != 3:
+                    raise ValueError("pol_in_{0} pol_out_{0} must be array of len 3".format(k))
+                pairs.append((a, b))
+        raman = np.asarray(raman, dtype=TYPE_DP)
+        if raman.ndim != 4 or raman.shape[1:3] != (3, 3) or raman.shape[3] != 3 * self.nat:
+            raise ValueError("The raman tensor in input have the wrong shape. The required is "
+                             "N_conf - E_field - E_field - 3 * N_at_sc")
+        u, rho = self._ensemble_arrays(ensemble, raman.shape[0])
+        ops = self._sym_ops(symmetrize)
+        d1 = pt.get_d1alpha_dR_av(rho, raman, self.device, symmetrize, ops)
+        v = sum(pt.raman_vector(d1, a, b) for a, b in pairs)
+        self.prepare_perturbation(v, masses_exp=-1)
+        if self.verbose:
+            print("Pertubation modulus with one ph effects only = {}".format(
+                self.perturbation_modulus))
+        if not add_two_ph:
+            return
+        if raman_eq is not None:
+            raman_eq = np.asarray(raman_eq, dtype=TYPE_DP)
+            if raman_eq.ndim != 3 or raman_eq.shape[:2] != (3, 3):
+                raise ValueError("Error, raman tns of the wrong shape: {}".format(raman_eq.shape))
+            n_sc = self._n_supercell(raman_eq.shape[2])
+            if not self.ignore_small_w and raman_eq.shape[2] * n_sc != self.nat * 3:
+                raise ValueError("the equilibrium Raman tensor does not match the supercell")
+            raman = raman - np.tile(raman_eq, n_sc)[None]
+        d2 = pt.get_d2alpha_dR_av(u, rho, raman, self._upsilon(), self.device,
+                                  symmetrize=symmetrize, symmetries=ops)
+        sm = 1.0 / np.sqrt(self.m)
+        d2 = d2 * sm[None, None, :, None] * sm[None, None, None, :]
+        if save_raman_tensor2:
+            np.save("{}".format(file_raman_tensor2), d2)
+            return
+        M = sum(np.einsum("abcd, a, b -> cd", d2, np.asarray(a, dtype=TYPE_DP),
+                          np.asarray(b, dtype=TYPE_DP)) for a, b in pairs)
+        self._set_two_phonon(self.pols.T @ M @ self.pols, " RAMAN")
+
+    def prepare_anharmonic_raman_FT_2ph(self, d2alpha_dR=None, pol_in=np.array([1., 0., 0.]),
+                                        pol_out=np.array([1., 0., 0.]), mixed=False,
+                                        pol_in_2=None, pol_out_2=None):
+        """Two-phonon sectors from a given second-order Raman tensor (3, 3, 3 nat_sc,
+        3 nat_sc), already divided by the masses (Py:1535-1644); the one-phonon sector of psi
+        is left as it is."""
+        if not self.use_wigner:
+            raise NotImplementedError("The two phonon processes are implemented only in Wigner")
+        if d2alpha_dR is None:
+            raise ValueError("Must specify the 2nd order Raman tensor!")
+        d2 = np.asarray(d2alpha_dR, dtype=TYPE_DP)
+        exp_shape = (3, 3, self.nat * 3, self.nat * 3)
+        if d2.shape != exp_shape:
+            raise ValueError("The shape of the 2nd order Raman tensor is not correct, expected "
+                             "{}".format(exp_shape))
+        pairs = [(pol_in, pol_out)]
+        if mixed:
+            if pol_in_2 is None or pol_out_2 is None:
+                raise ValueError("Must specify pol_in_2 pol_out_2 if mixed = True!")
+            if len(pol_in_2) != 3 or len(pol_out_2) != 3:
+                raise ValueError("pol_in_2 pol_out_2 must be array of len 3")
+            pairs.append((pol_in_2, pol_out_2))
+        M = sum(np.einsum("abcd, a, b -> cd", d2, np.asarray(a, dtype=TYPE_DP),
+                          np.asarray(b, dtype=TYPE_DP)) for a, b in pairs)
+        self._set_two_phonon(self.pols.T @ M @ self.pols, " RAMAN")
 
     # ------------------------------------------------------------------ psi helpers
     def get_Y1(self, half_off_diagonal=False):
