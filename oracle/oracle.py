@@ -32,6 +32,9 @@ Contents
   latter with CellConstructor (absent here): its exact form (the acoustic sum rule in
   particular) is parity unpinned; the restatement is pinned on invariance under every
   operation, idempotence, the sum rule and group-invariant model tensors.
+* The translation average of a psi vector (``translation_average_psi``): the projector on
+  translation-invariant perturbations, restated with explicit loops over the translations'
+  atom permutations; the checker of the zone-centre reduction of the image sum.
 * The derivative of a linear functional of the kernel outputs with respect to the
   configuration forces Y (``force_grad``), image by image in the reference's own form
   (u = S x_i, f = S y_i, C:2158-2170), and the adjoint seeds of the post-transform
@@ -837,3 +840,40 @@ def symmetrize_average(kind, C, rotations, irt, asr=True):
     if kind == "d2M":
         return S.reshape(3 * nat, 3 * nat, 3)
     return S.transpose(4, 5, 0, 1, 2, 3).reshape(3, 3, 3 * nat, 3 * nat)
+
+
+def translation_average_psi(psi, pols, irt_translations, adjoint=False):
+    """(1/n_t) sum_t Ad_t psi: R -> S_t R, each packed symmetric matrix M -> S_t M S_t^T with
+    S_t = E^T P_t E (P_t the Cartesian permutation of translation t, E the polarization
+    vectors of the kept modes). adjoint: the transpose in the packed metric, W P0 W^-1 with
+    W = 2 on the off-diagonal entries."""
+    E = np.asarray(pols, dtype=np.float64)
+    n = E.shape[1]
+    nw2 = n * (n + 1) // 2
+    iu = np.triu_indices(n)
+    w = np.ones(n + 2 * nw2)
+    off = iu[0] < iu[1]
+    w[n:n + nw2][off] = 2
+    w[n + nw2:][off] = 2
+    x = psi / w if adjoint else psi.copy()
+    perms = [(3 * np.asarray(p)[:, None] + np.arange(3)[None]).ravel() for p in irt_translations]
+    out = np.empty_like(x)
+    c = E @ x[:n]
+    acc = np.zeros_like(c)
+    for p in perms:
+        o = np.empty_like(c)
+        o[p] = c
+        acc += o
+    out[:n] = E.T @ acc / len(perms)
+    for k in range(2):
+        M = np.zeros((n, n))
+        M[iu] = x[n + k * nw2:n + (k + 1) * nw2]
+        M = M + np.triu(M, 1).T
+        C = E @ M @ E.T
+        acc = np.zeros_like(C)
+        for p in perms:
+            o = np.empty_like(C)
+            o[np.ix_(p, p)] = C
+            acc += o
+        out[n + k * nw2:n + (k + 1) * nw2] = (E.T @ acc @ E / len(perms))[iu]
+    return out * w if adjoint else out

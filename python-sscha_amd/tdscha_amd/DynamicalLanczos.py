@@ -101,6 +101,9 @@ class Lanczos(object):
         self.effective_charges = None   # (nat_uc, 3, 3), read by prepare_ir
         self.raman_tensor = None        # (3, 3, 3 nat_uc), read by prepare_raman
         self.sc_symmetries = None       # (rotations, irt) given to prepare_symmetrization
+        # translation-invariant perturbations (IR, Raman at the zone centre): L over one
+        # symmetry per rotation and the translation average (tdscha_amd.gamma); set before init
+        self.gamma_sector = False
         self.L_linop = None
         self._dev_ens = None
         if ensemble is not None and hasattr(ensemble, "u_disps"):
@@ -222,8 +225,19 @@ class Lanczos(object):
         sym = SymmetryBlocks(self.symmetries, self.N_degeneracy, self.degenerate_space,
                              self.sym_block_id)
         precs = ("fp64", _GPU_PRECISION[self.mode]) if self.mode != MODE_GPU_FP64 else ("fp64",)
-        self._dev_ens = DeviceEnsemble(self.X, self.Y, self.w, self.rho, self.T, sym,
-                                       precisions=precs, device=self.device, s_batch=self.s_batch)
+        if self.gamma_sector:
+            from .gamma import GammaSectorOperator
+            if self.sc_symmetries is None or not use_symmetries:
+                raise ValueError("gamma_sector needs the space group: call "
+                                 "prepare_symmetrization(symmetries=(rotations, irt)) first")
+            rot, irt = self.sc_symmetries
+            self._dev_ens = GammaSectorOperator(self.X, self.Y, self.w, self.rho, self.T, sym,
+                                                rot, irt, self.pols, precisions=precs,
+                                                device=self.device, s_batch=self.s_batch)
+        else:
+            self._dev_ens = DeviceEnsemble(self.X, self.Y, self.w, self.rho, self.T, sym,
+                                           precisions=precs, device=self.device,
+                                           s_batch=self.s_batch)
         self.initialized = True
         import scipy.sparse.linalg
         n_psi = len(self.psi)
@@ -703,6 +717,13 @@ class Lanczos(object):
         if i_step == 0:
             first = to_dev(psi_host)
             first = first / torch.sqrt(dot(first, first))
+            if self.gamma_sector:
+                for adj in ((False,) if run_simm or self.use_wigner else (False, True)):
+                    r = ens.proj.residual(first, adjoint=adj)
+                    if r > 1e-8:
+                        raise ValueError("gamma_sector: the perturbation is not "
+                                         "translation-invariant (|psi - P0 psi| / |psi| = %g)"
+                                         % r)
             Q, P, s_norm_l = [first], [first.clone()], [1.0]
         else:
             Q = [to_dev(q) for q in self.basis_Q]
@@ -832,6 +853,10 @@ class Lanczos(object):
     def _run_device_loop(self, n_iter, run_simm, graph, save_dir, prefix):
         from .lanczos_device import DeviceLanczos
         ens = self.device_ensemble
+        if self.gamma_sector:
+            raise NotImplementedError("device_loop=True runs the recurrence on the full "
+                                      "ensemble's graphs; use device_loop=False with "
+                                      "gamma_sector")
         if run_simm and not self.use_wigner:
             raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
         psi = np.asarray(self.psi, dtype=TYPE_DP)

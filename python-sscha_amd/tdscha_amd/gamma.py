@@ -1,0 +1,181 @@
+"""L on translation-invariant (Gamma-point) perturbations over one symmetry per rotation.
+
+The symmetrized ensemble average (the sum over the N_sym images of every configuration that
+Modules/DynamicalLanczos.py:3051-3106 hands to CModules/LanczosFunctions.c:2082-2754) runs over
+a space group: N_sym = |P| |T| operations, the cosets g = t o p of the pure translations T
+(rotation = identity) by one operation p per rotation. When the perturbation is
+translation-invariant, every image of a coset sees the same rotated operands
+(S_g^T Y1 S_g = S_p^T Y1 S_p because S_t^T Y1 S_t = Y1), and the coset's outputs are the
+translates of the representative's. So
+
+    L psi = P0 (L_P psi)
+
+with L_P the same operator over the |P| representatives only and P0 the average over the
+translations (the projector on the translation-invariant sector). IR and Raman perturbations
+at the zone centre are in that sector and L keeps them there, so their Lanczos chains need
+|T| times fewer images (216 for the 6 x 6 x 6 rock-salt supercell: 48 instead of 10368).
+
+P0 runs in Cartesian space, where a translation is a permutation of the atoms:
+S_t M S_t^T = E^T P_t (E M E^T) P_t^T E for the polarization vectors E of the kept modes. psi
+holds upper triangles of symmetric matrices, whose dot product counts an off-diagonal pair
+once; in that metric the transpose of L is covariant with P0^T = W P0 W^-1 (W = 2 on the
+off-diagonal entries), which is the projector for a transposed application in the standard
+representation. In the Wigner representation both directions use P0.
+"""
+import numpy as np
+
+from . import _lib as L
+from .ensemble import DeviceEnsemble, SymmetryBlocks
+from .symmetry import translation_cosets
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class TranslationProjector:
+    """P0 and its packed adjoint on device vectors psi = (R (n), two packed n x n upper
+    triangles). pols: (3 nat, n) polarization vectors of the kept modes (orthonormal
+    columns); irt_translations: (n_t, nat) the atom each pure translation carries every atom
+    to."""
+
+    def __init__(self, pols, irt_translations, device=0, chunk=16):
+        torch = _torch()
+        self.torch = torch
+        self.dev = torch.device("cuda", device) if isinstance(device, int) else device
+        E = np.ascontiguousarray(pols, dtype=np.float64)
+        if E.ndim != 2 or E.shape[0] % 3:
+            raise ValueError("pols must be (3 nat, n_modes)")
+        irt = np.asarray(irt_translations, dtype=np.int64)
+        nat = E.shape[0] // 3
+        if irt.ndim != 2 or irt.shape[1] != nat:
+            raise ValueError("irt_translations must be (n_t, %d)" % nat)
+        if np.any(np.sort(irt, axis=1) != np.arange(nat)[None]):
+            raise ValueError("every row of irt_translations must be a permutation of the atoms")
+        self.n = E.shape[1]
+        self.E = torch.as_tensor(E, device=self.dev)
+        cart = (3 * irt[:, :, None] + np.arange(3)[None, None]).reshape(len(irt), -1)
+        self.perm = torch.as_tensor(cart, device=self.dev)           # (n_t, 3 nat)
+        self.chunk = chunk
+        n = self.n
+        iu = np.triu_indices(n)
+        self.iu = (torch.as_tensor(iu[0], device=self.dev), torch.as_tensor(iu[1], device=self.dev))
+        w = np.where(iu[0] < iu[1], 2.0, 1.0)
+        self.w_off = torch.as_tensor(w, device=self.dev)
+
+    # the average over the group is the same through a permutation or its inverse
+    def _avg_vec(self, c):
+        return c[self.perm].mean(0)
+
+    def _avg_mat(self, C):
+        acc = self.torch.zeros_like(C)
+        for t0 in range(0, self.perm.shape[0], self.chunk):
+            p = self.perm[t0:t0 + self.chunk]
+            acc += C[p[:, :, None], p[:, None, :]].sum(0)
+        return acc / self.perm.shape[0]
+
+    def vector(self, r):
+        return self.E.T @ self._avg_vec(self.E @ r)
+
+    def matrix(self, M):
+        return self.E.T @ self._avg_mat(self.E @ M @ self.E.T) @ self.E
+
+    def project(self, psi, adjoint=False, out=None):
+        """P0 psi, or P0^T psi (adjoint=True)."""
+        torch = self.torch
+        n = self.n
+        nw2 = n * (n + 1) // 2
+        if psi.shape[0] != n + 2 * nw2:
+            raise ValueError("psi has %d entries, expected %d" % (psi.shape[0], n + 2 * nw2))
+        if out is None:
+            out = torch.empty_like(psi)
+        out[:n] = self.vector(psi[:n])
+        for k in range(2):
+            v = psi[n + k * nw2:n + (k + 1) * nw2]
+            if adjoint:
+                v = v / self.w_off
+            M = torch.zeros((n, n), dtype=psi.dtype, device=psi.device)
+            M[self.iu] = v
+            M = M + M.triu(1).T
+            r = self.matrix(M)[self.iu]
+            out[n + k * nw2:n + (k + 1) * nw2] = r * self.w_off if adjoint else r
+        return out
+
+    def residual(self, psi, adjoint=False):
+        """|psi - P psi| / |psi|: how far psi is from the sector."""
+        d = self.project(psi, adjoint) - psi
+        return float(self.torch.linalg.norm(d) / self.torch.linalg.norm(psi))
+
+
+class GammaSectorOperator:
+    """L on translation-invariant perturbations: a DeviceEnsemble over one symmetry per
+    rotation, followed by the translation average. Same apply_full_L / apply_full_L_batched
+    interface as DeviceEnsemble; results equal the full operator's for psi in the sector
+    (transposed standard-representation applications: psi in the sector of P0^T).
+
+    sym: the symmetry blocks of all N_sym operations, in the order of (rotations, irt), the
+    supercell's space group; pols: (3 nat, n) polarization vectors of the kept modes."""
+
+    def __init__(self, X, Y, w, rho, T, sym: SymmetryBlocks, rotations, irt, pols,
+                 precisions=("fp64",), device=0, s_batch=0, check_tol=None):
+        cos = translation_cosets(rotations, irt)
+        if cos is None:
+            raise ValueError("the operations are not cosets of the pure translations (a space "
+                             "group with at least two translations is needed)")
+        reps, trans = cos
+        if sym.symmetries[0].shape[0] != len(rotations):
+            raise ValueError("the symmetry blocks hold %d operations, the space group %d"
+                             % (sym.symmetries[0].shape[0], len(rotations)))
+        self.representatives = reps
+        self.translations = trans
+        sub = SymmetryBlocks([np.ascontiguousarray(s[reps]) for s in sym.symmetries],
+                             sym.N_degeneracy, sym.degenerate_space, sym.block_id)
+        self.ens = DeviceEnsemble(X, Y, w, rho, T, sub, precisions=precisions, device=device,
+                                  s_batch=s_batch)
+        self.proj = TranslationProjector(pols, np.asarray(irt)[trans], device=device)
+        self.n = self.ens.n
+        self.device = device
+        self.torch = self.ens.torch
+        self.check_tol = check_tol
+
+    @staticmethod
+    def _adjoint(flags):
+        return bool(flags & L.OP_TRANSPOSE) and not bool(flags & L.OP_WIGNER)
+
+    def _check(self, psi, flags):
+        if self.check_tol is not None:
+            r = self.proj.residual(psi, self._adjoint(flags))
+            if r > self.check_tol:
+                raise ValueError("psi is not translation-invariant (|psi - P0 psi| / |psi| = %g):"
+                                 " use the full operator" % r)
+
+    def apply_full_L(self, psi, flags=0, shift=0.0, precision="fp64", out=None):
+        psi = self.ens._dev(psi, (self.n + self.n * (self.n + 1),))
+        self._check(psi, flags)
+        tmp = self.ens.apply_full_L(psi, flags, shift, precision)
+        return self.proj.project(tmp, self._adjoint(flags), out=out)
+
+    def apply_full_L_batched(self, psis, flags=None, shifts=None, precision="fp64", outs=None):
+        C = len(psis)
+        flags = [0] * C if flags is None else list(flags)
+        psis = [self.ens._dev(p, (self.n + self.n * (self.n + 1),)) for p in psis]
+        for p, f in zip(psis, flags):
+            self._check(p, f)
+        tmp = self.ens.apply_full_L_batched(psis, flags, shifts, precision)
+        if outs is None:
+            outs = [None] * C
+        return [self.proj.project(t, self._adjoint(f), out=o)
+                for t, f, o in zip(tmp, flags, outs)]
+
+    def autotune(self, precision="bf16x3", n_chains=2):
+        return self.ens.autotune(precision, n_chains)
+
+    def set_tuning(self, bf16_tiles=0, pass_slots=0):
+        return self.ens.set_tuning(bf16_tiles, pass_slots)
+
+    def tuning(self):
+        return self.ens.tuning()
+
+    def close(self):
+        self.ens.close()

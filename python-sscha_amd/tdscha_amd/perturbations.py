@@ -221,31 +221,6 @@ def _act(T, R, inv, n_atom_axes, cart):
     return X
 
 
-def _cosets(rot, irt):
-    """Split the operations into cosets of the pure translations (rotation = identity): a
-    list of (representative, translations) when every rotation appears once per translation
-    and the operations with one rotation are exactly the representative composed with each
-    translation (as for a space group); None otherwise."""
-    rot = np.asarray(rot)
-    eye = np.abs(rot - np.eye(3)[None]).max(axis=(1, 2)) < 1e-10
-    tr = np.nonzero(eye)[0]
-    if len(tr) < 2 or len(rot) % len(tr):
-        return None
-    key = np.round(rot.reshape(len(rot), 9), 8)
-    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
-    inverse = inverse.ravel()
-    reps = []
-    for k, g in enumerate(first):
-        members = np.nonzero(inverse == k)[0]
-        if len(members) != len(tr):
-            return None
-        composed = irt[g][irt[tr]]                  # irt_g o irt_t for every translation t
-        if not np.array_equal(np.unique(composed, axis=0), np.unique(irt[members], axis=0)):
-            return None
-        reps.append(g)
-    return np.array(reps), tr
-
-
 def space_group_average(Tc, n_atom_axes, symmetries, device=None):
     """(1 / n_sym) sum_g g.T for a tensor in canonical layout: n_atom_axes pairs (nat, 3)
     first, then field axes of length 3. g = (R, irt) acts as
@@ -260,7 +235,8 @@ def space_group_average(Tc, n_atom_axes, symmetries, device=None):
     rot, inv = _ops(symmetries, nat, dev)
     irt = np.asarray(symmetries[1], dtype=np.int64)
     cart = [2 * k + 1 for k in range(n_atom_axes)] + list(range(2 * n_atom_axes, T.dim()))
-    cos = _cosets(symmetries[0], irt)
+    from .symmetry import translation_cosets
+    cos = translation_cosets(symmetries[0], irt)
     if cos is None:
         acc = torch.zeros_like(T)
         for g in range(rot.shape[0]):

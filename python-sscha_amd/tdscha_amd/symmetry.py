@@ -198,3 +198,32 @@ def _main():
 
 if __name__ == "__main__":
     _main()
+
+
+def translation_cosets(rotations, irt):
+    """Split space-group operations into cosets of the pure translations (rotation = identity).
+
+    Returns (representatives, translations): one operation index per distinct rotation and the
+    indices of the pure translations, when every rotation appears once per translation and
+    the operations with one rotation are exactly the representative composed with each
+    translation (irt_g o irt_t), as for a space group; None otherwise (then no reduction
+    applies)."""
+    rot = np.asarray(rotations, dtype=np.float64)
+    irt = np.asarray(irt, dtype=np.int64)
+    eye = np.abs(rot - np.eye(3)[None]).max(axis=(1, 2)) < 1e-10
+    tr = np.nonzero(eye)[0]
+    if len(tr) < 2 or len(rot) % len(tr):
+        return None
+    key = np.round(rot.reshape(len(rot), 9), 8)
+    _, first, inverse = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inverse = inverse.ravel()
+    reps = []
+    for k, g in enumerate(first):
+        members = np.nonzero(inverse == k)[0]
+        if len(members) != len(tr):
+            return None
+        composed = irt[g][irt[tr]]                  # irt_g o irt_t for every translation t
+        if not np.array_equal(np.unique(composed, axis=0), np.unique(irt[members], axis=0)):
+            return None
+        reps.append(g)
+    return np.sort(np.array(reps)), tr

@@ -1,0 +1,131 @@
+"""Zone-centre reduction on the device (tdscha_amd.gamma.GammaSectorOperator) against the full
+device operator over all images, and against the oracle's translation average."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def ops():
+    from tdscha_amd import synthetic as sy
+    from tdscha_amd.ensemble import DeviceEnsemble, SymmetryBlocks
+    from tdscha_amd.gamma import GammaSectorOperator
+    print("case: 2x2x2 rock salt, N=300, 384 operations", flush=True)
+    ens = sy.make_ensemble(2, 300, T=250.0, weights="lognormal")
+    rots, irt = sy.space_group(ens.crystal)
+    sym = SymmetryBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    args = (ens.X, ens.Y, ens.w, ens.rho, ens.T, sym)
+    full = DeviceEnsemble(*args, precisions=("fp64", "bf16x3"))
+    gam = GammaSectorOperator(*args, rots, irt, ens.pols, precisions=("fp64", "bf16x3"))
+    return ens, rots, irt, full, gam
+
+
+def _sector_psi(ens, irt, tr, seed, adjoint):
+    n = len(ens.w)
+    v = np.random.default_rng(seed).standard_normal(n + n * (n + 1))
+    return orc.translation_average_psi(v, ens.pols, irt[tr], adjoint)
+
+
+def test_projector_matches_oracle(ops):
+    import torch
+    ens, rots, irt, full, gam = ops
+    n = len(ens.w)
+    v = np.random.default_rng(3).standard_normal(n + n * (n + 1))
+    for adj in (False, True):
+        got = gam.proj.project(torch.as_tensor(v, device="cuda"), adjoint=adj).cpu().numpy()
+        expect = orc.translation_average_psi(v, ens.pols, irt[gam.translations], adj)
+        assert rel(got, expect) < 1e-13
+
+
+@pytest.mark.parametrize("wigner", [False, True])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_gamma_operator_equals_full_operator(ops, wigner, transpose):
+    from tdscha_amd import _lib as L
+    ens, rots, irt, full, gam = ops
+    print("case: wigner=%s transpose=%s" % (wigner, transpose), flush=True)
+    fl = (L.OP_WIGNER if wigner else 0) | (L.OP_TRANSPOSE if transpose else 0)
+    psi = _sector_psi(ens, irt, gam.translations, 4, transpose and not wigner)
+    a = full.apply_full_L(psi, flags=fl, shift=0.3).cpu().numpy()
+    b = gam.apply_full_L(psi, flags=fl, shift=0.3).cpu().numpy()
+    assert rel(b, a) < 1e-12
+
+
+def test_gamma_operator_batched_chains(ops):
+    from tdscha_amd import _lib as L
+    ens, rots, irt, full, gam = ops
+    print("case: batched L q, L^T p", flush=True)
+    q = _sector_psi(ens, irt, gam.translations, 5, False)
+    p = _sector_psi(ens, irt, gam.translations, 6, True)
+    fl = [0, L.OP_TRANSPOSE]
+    a = [x.cpu().numpy() for x in full.apply_full_L_batched([q, p], flags=fl)]
+    b = [x.cpu().numpy() for x in gam.apply_full_L_batched([q, p], flags=fl)]
+    assert rel(b[0], a[0]) < 1e-12 and rel(b[1], a[1]) < 1e-12
+
+
+def test_gamma_operator_bf16x3(ops):
+    ens, rots, irt, full, gam = ops
+    print("case: bf16x3", flush=True)
+    psi = _sector_psi(ens, irt, gam.translations, 7, False)
+    ref = full.apply_full_L(psi, precision="fp64").cpu().numpy()
+    a = full.apply_full_L(psi, precision="bf16x3").cpu().numpy()
+    b = gam.apply_full_L(psi, precision="bf16x3").cpu().numpy()
+    # the reduced sum carries the same BF16x3 error scale as the full one
+    print("bf16x3 error: full %.3g, reduced %.3g" % (rel(a, ref), rel(b, ref)), flush=True)
+    assert rel(b, ref) < 1e-4
+    assert rel(b, ref) < 4 * rel(a, ref) + 1e-9
+
+
+def _lanczos(ens, rots, irt, gamma, wigner=False):
+    from tdscha_amd import DynamicalLanczos as DL
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64, use_wigner=wigner)
+    lz.verbose = False
+    lz.set_ensemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, pols=ens.pols, m=ens.m)
+    lz.prepare_symmetrization(symmetries=(rots, irt), verbose=False)
+    lz.gamma_sector = gamma
+    lz.init()
+    lz.prepare_ir(effective_charges=np.stack([2 * np.eye(3), -2 * np.eye(3)]),
+                  pol_vec=np.array([1.0, 0.0, 0.0]))
+    return lz
+
+
+@pytest.mark.parametrize("start", ["ir", "wigner_random"])
+def test_lanczos_in_the_gamma_sector_reproduces_the_full_run(ops, start):
+    """Biconjugate Lanczos from the IR perturbation, and the Wigner symmetric one from a random
+    translation-invariant perturbation with two-phonon parts (10 steps)."""
+    ens, rots, irt, full, gam = ops
+    print("case: run_FT, %s start" % start, flush=True)
+    runs = []
+    for gamma in (False, True):
+        lz = _lanczos(ens, rots, irt, gamma, wigner=start != "ir")
+        if start != "ir":
+            lz.psi = _sector_psi(ens, irt, gam.translations, 9, False)
+            lz.run_FT(10, verbose=False, run_simm=True)
+        else:
+            lz.run_FT(10, verbose=False)
+        runs.append(lz)
+    for k in ("a_coeffs", "b_coeffs", "c_coeffs"):
+        a = np.array(getattr(runs[0], k))
+        b = np.array(getattr(runs[1], k))
+        assert len(a) == len(b), k
+        if k == "a_coeffs":
+            assert len(a) >= 1
+        if start != "ir" and k == "a_coeffs":
+            assert len(a) == 10
+        if len(a):
+            assert rel(b, a) < 1e-10, k
+
+
+def test_lanczos_gamma_sector_refuses_a_perturbation_outside_the_sector(ops):
+    ens, rots, irt, full, gam = ops
+    print("case: run_FT gamma_sector with a random perturbation", flush=True)
+    lz = _lanczos(ens, rots, irt, True)
+    lz.psi = np.random.default_rng(8).standard_normal(len(lz.psi))
+    with pytest.raises(ValueError, match="translation-invariant"):
+        lz.run_FT(2, verbose=False)

@@ -237,8 +237,9 @@ def test_symmetrize_average_of_a_non_group_list_matches_oracle():
     print("case: symmetrize d2M over 100 of the 384 operations", flush=True)
     _, _, E, _, _, rots, irt = tp.rock_salt_2()
     n3 = E.shape[0]
-    assert pt._cosets(rots[:100], irt[:100]) is None
-    assert pt._cosets(rots, irt) is not None
+    from tdscha_amd.symmetry import translation_cosets
+    assert translation_cosets(rots[:100], irt[:100]) is None
+    assert translation_cosets(rots, irt) is not None
     C = np.random.default_rng(8).standard_normal((n3, n3, 3))
     got = pt.symmetrize_average("d2M", C, (rots[:100], irt[:100]), device=0).cpu().numpy()
     expect = orc.symmetrize_average("d2M", C, rots[:100], irt[:100])
