@@ -19,6 +19,12 @@ executes (GEMM2 forms B_s once and uses B_s + B_s^T, half its algorithmic count)
 `cpu_baseline` times the reference's own C trio
 (oracle/_ref, built from CModules/LanczosFunctions.c) on one host core over a bounded
 sample of images of the same ensemble and scales it linearly to the full image count.
+
+`gamma_sector` is a separate line, not the headline: the same two chains projected on the
+translation-invariant sector (the zone-centre perturbations of IR and Raman spectra live
+there) and applied by tdscha_amd.gamma.GammaSectorOperator, which sums one symmetry per
+rotation (48) and averages the result over the 216 translations; it is checked once against
+the full operator on the same vectors (max_rel_diff_vs_full_operator).
 """
 import argparse
 import json
@@ -105,6 +111,54 @@ def cpu_baseline(ens, seconds=12.0):
                           k, t, per_image, ens.X.shape[0], ens.n_syms)}
 
 
+def gamma_sector_line(args, ens, sym, precs, dens, psis, flags, shifts, full_value, reps=5):
+    """The same chains projected on the translation-invariant sector (where IR and Raman
+    perturbations at the zone centre live) and applied by the reduced operator (one symmetry
+    per rotation, then the translation average); checked once against the full operator on
+    the same vectors. A separate line: the headline value is the full operator on general
+    vectors."""
+    import torch
+    from tdscha_amd import _lib as L
+    from tdscha_amd import synthetic as sy
+    from tdscha_amd.gamma import GammaSectorOperator
+    rots, irt = sy.space_group(ens.crystal)
+    t0 = time.time()
+    gop = GammaSectorOperator(ens.X, ens.Y, ens.w, ens.rho, ens.T, sym, rots, irt, ens.pols,
+                              precisions=precs, s_batch=args.s_batch)
+    if args.precision == "bf16x3" and not args.no_autotune:
+        gop.autotune(args.precision, args.chains)
+    vs = []
+    for p, f in zip(psis, flags):
+        v = gop.proj.project(p, adjoint=bool(f & L.OP_TRANSPOSE))
+        vs.append(v / v.norm())
+    outs = [torch.empty_like(v) for v in vs]
+    gop.apply_full_L_batched(vs, flags, shifts, precision=args.precision, outs=outs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        gop.apply_full_L_batched(vs, flags, shifts, precision=args.precision, outs=outs)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    rate = args.chains * reps / el
+    ref = dens.apply_full_L_batched(vs, flags, shifts, precision=args.precision)
+    torch.cuda.synchronize()
+    diff = max(float((o - r).abs().max() / r.abs().max()) for o, r in zip(outs, ref))
+    log("zone-centre operator: %.4g L/s (%d representatives x %d translations), %.0fx the "
+        "full operator, max rel diff %.2g, set-up %.1f s" % (
+            rate, len(gop.representatives), len(gop.translations), rate / full_value, diff,
+            time.time() - t0))
+    res = {"value": rate, "unit": "L-applications/s",
+           "sample": "the %d chains projected on the translation-invariant sector, %d batched "
+                     "applications (%.3f s) over %d symmetries (one per rotation) and the "
+                     "average over %d translations" % (args.chains, reps, el,
+                                                       len(gop.representatives),
+                                                       len(gop.translations)),
+           "speedup_vs_full_operator": rate / full_value,
+           "max_rel_diff_vs_full_operator": diff}
+    gop.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +181,8 @@ def main():
                          "(a kernel of known traffic for hardware-counter calibration)")
     ap.add_argument("--fp64-sample-syms", type=int, default=-1,
                     help="symmetries of the FP64 comparison sample (-1: N_sym/32, 0: skip)")
+    ap.add_argument("--no-gamma", action="store_true",
+                    help="skip the zone-centre (translation-invariant) operator line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -233,6 +289,13 @@ def main():
     roofline = kernel_roofline(dom)
     roofline_other = kernel_roofline("gemm2" if dom == "gemm1" else "gemm1")
     value = args.chains * args.steps / elapsed
+    gamma = None
+    if not args.no_gamma:
+        try:
+            gamma = gamma_sector_line(args, ens, sym, precs, dens, psis, flags, shifts, value)
+        except (ValueError, RuntimeError) as e:     # an auxiliary line: the headline stands
+            log("zone-centre operator line failed: %s" % e)
+            gamma = {"error": str(e)}
     fp64_ref = None
     if args.precision != "fp64" and args.fp64_sample_syms != 0:
         # the plain FP64 path on the first k symmetries: its cost is linear in the number of
@@ -288,6 +351,7 @@ def main():
         "roofline_other_contraction": roofline_other,
         "cpu_baseline": base,
         "fp64_plain": fp64_ref,
+        "gamma_sector": gamma,
     }
     print(json.dumps(res), flush=True)
 
