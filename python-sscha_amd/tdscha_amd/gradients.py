@@ -33,7 +33,10 @@ class SpectralProblem:
     frequencies w_k (Ry), the smearing (Ry), the number of Lanczos steps and the mode."""
 
     def __init__(self, X, w, rho, T, sym, omegas, smearing, n_steps, mode=MODE_GPU_FP64,
-                 full_gs=True):
+                 full_gs=True, gamma_sector=None, verbose=False):
+        """gamma_sector: (rotations, irt, pols) of the supercell's space group to run the
+        Lanczos and the adjoint on the zone-centre operator (tdscha_amd.gamma), for
+        translation-invariant psi0 such as IR perturbations."""
         self.X = np.ascontiguousarray(X, dtype=np.float64)
         self.w = np.ascontiguousarray(w, dtype=np.float64)
         self.rho = np.ascontiguousarray(rho, dtype=np.float64)
@@ -44,16 +47,22 @@ class SpectralProblem:
         self.n_steps = int(n_steps)
         self.mode = mode
         self.full_gs = bool(full_gs)
+        self.gamma_sector = gamma_sector
+        self.verbose = bool(verbose)
 
     def lanczos(self, psi0, Y):
         lz = Lanczos(mode=self.mode)
-        lz.set_ensemble(self.X, Y, self.w, self.rho, self.T)
+        pols = None if self.gamma_sector is None else self.gamma_sector[2]
+        lz.set_ensemble(self.X, Y, self.w, self.rho, self.T, pols=pols)
         s = self.sym
         lz.set_symmetries(s.symmetries, s.N_degeneracy, s.degenerate_space, s.block_id)
+        if self.gamma_sector is not None:
+            lz.sc_symmetries = tuple(self.gamma_sector[:2])
+            lz.gamma_sector = True
         lz.init()
         lz.psi[:] = psi0
         lz.perturbation_modulus = float(np.dot(psi0, psi0))
-        lz.run_FT(self.n_steps, verbose=False, full_gs=self.full_gs)
+        lz.run_FT(self.n_steps, verbose=self.verbose, full_gs=self.full_gs)
         return lz
 
     def value(self, lz):
@@ -124,6 +133,9 @@ def spectral_gradients(problem, psi0, Y, pairs_per_call=32):
         j1 = min(k, j0 + pairs_per_call)
         ens.force_grad([U[j] for j in range(j0, j1)], [Q[j] for j in range(j0, j1)],
                        out=gY, accumulate=True)
+        if problem.verbose:
+            torch.cuda.synchronize()
+            print("force adjoint: pairs %d-%d of %d done" % (j0, j1 - 1, k), flush=True)
     return J, g_psi, -gY.cpu().numpy()
 
 
@@ -149,3 +161,62 @@ def spectral_function_sum(problem, psi0, Y):
 
     return _SpectralSum.apply(torch.as_tensor(psi0, dtype=torch.float64),
                               torch.as_tensor(Y, dtype=torch.float64))
+
+
+def _main():
+    """Time J = sum_k Im G(w_k) and its gradients on the synthetic rock-salt ensemble:
+    python -m tdscha_amd.gradients [--L 4] [--configs 10000] [--steps 150] [--anharmonic 20]
+    IR perturbation along x (Z* = +-2 I), 64 frequencies over 50-250 cm^-1 with 1 cm^-1
+    smearing, FP64; the full operator and the zone-centre one (tdscha_amd.gamma), and the
+    largest differences between the two."""
+    import argparse
+    import json
+    import time
+
+    import torch
+
+    from . import synthetic as sy
+    from .ensemble import SymmetryBlocks
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--L", type=int, default=4)
+    ap.add_argument("--configs", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=150)
+    ap.add_argument("--anharmonic", type=float, default=20.0)
+    ap.add_argument("--skip-full", action="store_true")
+    a = ap.parse_args()
+    ens = sy.make_ensemble(a.L, a.configs, T=300.0, weights="lognormal",
+                           device=torch.device("cuda", 0), sigma3=0.1 * a.anharmonic,
+                           sigma4=0.05 * a.anharmonic)
+    rots, irt = sy.space_group(ens.crystal)
+    sym = SymmetryBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    omegas = np.linspace(50.0, 250.0, 64) / sy.RY_TO_CM
+    n = len(ens.w)
+    psi0 = np.zeros(n + n * (n + 1))
+    psi0[:n] = ens.ir_perturbation(0)
+    res = {"workload": "rock-salt %dx%dx%d, %d modes, %d configs, %d symmetries, IR, %d steps"
+                       % (a.L, a.L, a.L, n, a.configs, ens.n_syms, a.steps)}
+    out = {}
+    for name, gam in (("full", None), ("zone_centre", (rots, irt, ens.pols))):
+        if name == "full" and a.skip_full:
+            continue
+        prob = SpectralProblem(ens.X, ens.w, ens.rho, ens.T, sym, omegas, 1.0 / sy.RY_TO_CM,
+                               a.steps, gamma_sector=gam, verbose=True)
+        t0 = time.time()
+        J, gp, gy = spectral_gradients(prob, psi0, ens.Y)
+        torch.cuda.synchronize()
+        out[name] = (J, gp, gy)
+        res[name] = {"seconds": time.time() - t0, "J": J}
+        print("%s: forward and backward done" % name, flush=True)
+        print("%s: %.1f s" % (name, res[name]["seconds"]), flush=True)
+    if "full" in out:
+        J0, gp0, gy0 = out["full"]
+        J1, gp1, gy1 = out["zone_centre"]
+        res["rel_diff"] = {"J": abs(J1 - J0) / abs(J0),
+                           "dJ_dpsi0": float(np.abs(gp1 - gp0).max() / np.abs(gp0).max()),
+                           "dJ_dY": float(np.abs(gy1 - gy0).max() / np.abs(gy0).max())}
+        res["speedup"] = res["full"]["seconds"] / res["zone_centre"]["seconds"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
