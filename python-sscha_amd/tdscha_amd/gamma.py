@@ -16,7 +16,10 @@ at the zone centre are in that sector and L keeps them there, so their Lanczos c
 |T| times fewer images (216 for the 6 x 6 x 6 rock-salt supercell: 48 instead of 10368).
 
 P0 runs in Cartesian space, where a translation is a permutation of the atoms:
-S_t M S_t^T = E^T P_t (E M E^T) P_t^T E for the polarization vectors E of the kept modes. psi
+S_t M S_t^T = E^T P_t (E M E^T) P_t^T E for the polarization vectors E of the kept modes; the
+average over the translations of a Cartesian matrix depends only on the orbits of its two
+atoms and their relative translation (the translations commute and act freely), so it is one
+scatter over the atom pairs and one gather back. psi
 holds upper triangles of symmetric matrices, whose dot product counts an off-diagonal pair
 once; in that metric the transpose of L is covariant with P0^T = W P0 W^-1 (W = 2 on the
 off-diagonal entries), which is the projector for a transposed application in the standard
@@ -32,6 +35,38 @@ from .symmetry import translation_cosets
 def _torch():
     import torch
     return torch
+
+
+def _orbit_fold(irt):
+    """For pure translations (an abelian group acting freely on the atoms): the orbit s(a) of
+    every atom and, for every atom pair, the index (s(a) nb + s(b)) n_t + d(a, b) with d the
+    translation tau_a^-1 tau_b (tau_x carries the orbit's first atom to x). None when the
+    action is not free."""
+    nt, nat = irt.shape
+    if nat % nt:
+        return None
+    s = -np.ones(nat, dtype=np.int64)
+    tau = -np.ones(nat, dtype=np.int64)
+    reps = []
+    for a in range(nat):
+        if s[a] >= 0:
+            continue
+        img = irt[:, a]
+        if len(np.unique(img)) != nt or np.any(s[img] >= 0):
+            return None
+        s[img] = len(reps)
+        tau[img] = np.arange(nt)
+        reps.append(a)
+    r0 = reps[0]
+    key = -np.ones(nat, dtype=np.int64)
+    key[irt[:, r0]] = np.arange(nt)
+    inv = np.argsort(irt, axis=1)
+    img = inv[tau[:, None], irt[tau[None, :], r0]]       # (tau_a^-1 tau_b)(r0)
+    d = key[img]
+    if np.any(d < 0):
+        return None
+    nb = len(reps)
+    return s, (s[:, None] * nb + s[None, :]) * nt + d, nb
 
 
 class TranslationProjector:
@@ -58,6 +93,20 @@ class TranslationProjector:
         cart = (3 * irt[:, :, None] + np.arange(3)[None, None]).reshape(len(irt), -1)
         self.perm = torch.as_tensor(cart, device=self.dev)           # (n_t, 3 nat)
         self.chunk = chunk
+        self.nat, self.nt = nat, len(irt)
+        self.fold = None
+        fold = _orbit_fold(irt)
+        if fold is not None:
+            s_orb, idx, nb = fold
+            self.s_orb = torch.as_tensor(s_orb, device=self.dev)
+            self.idx = torch.as_tensor(idx.ravel(), device=self.dev)
+            self.nb = nb
+            C = torch.randn(3 * nat, 3 * nat, dtype=torch.float64, device=self.dev)
+            r = torch.randn(3 * nat, dtype=torch.float64, device=self.dev)
+            self.fold = True
+            ok = (torch.allclose(self._avg_mat(C), self._avg_mat_gather(C), rtol=0, atol=1e-12)
+                  and torch.allclose(self._avg_vec(r), r[self.perm].mean(0), rtol=0, atol=1e-12))
+            self.fold = True if ok else None
         n = self.n
         iu = np.triu_indices(n)
         self.iu = (torch.as_tensor(iu[0], device=self.dev), torch.as_tensor(iu[1], device=self.dev))
@@ -66,9 +115,25 @@ class TranslationProjector:
 
     # the average over the group is the same through a permutation or its inverse
     def _avg_vec(self, c):
-        return c[self.perm].mean(0)
+        if self.fold is None:
+            return c[self.perm].mean(0)
+        V = self.torch.zeros((self.nb, 3), dtype=c.dtype, device=c.device)
+        V.index_add_(0, self.s_orb, c.view(self.nat, 3))
+        return (V[self.s_orb] / self.nt).reshape(-1)
 
     def _avg_mat(self, C):
+        """(1/n_t) sum_t C[t(a)][t(b)] = G[s(a)][s(b)][tau_a^-1 tau_b]: one scatter over the
+        atom pairs into (orbit, orbit, relative translation) and one gather back."""
+        if self.fold is None:
+            return self._avg_mat_gather(C)
+        nat = self.nat
+        B = C.view(nat, 3, nat, 3).permute(0, 2, 1, 3).reshape(nat * nat, 9)
+        G = self.torch.zeros((self.nb * self.nb * self.nt, 9), dtype=C.dtype, device=C.device)
+        G.index_add_(0, self.idx, B)
+        A = (G[self.idx] / self.nt).view(nat, nat, 3, 3)
+        return A.permute(0, 2, 1, 3).reshape(3 * nat, 3 * nat)
+
+    def _avg_mat_gather(self, C):
         acc = self.torch.zeros_like(C)
         for t0 in range(0, self.perm.shape[0], self.chunk):
             p = self.perm[t0:t0 + self.chunk]
