@@ -259,3 +259,69 @@ class GammaSectorOperator:
 
     def close(self):
         self.ens.close()
+
+
+def _main():
+    """Time the zone-centre operator on the synthetic rock-salt ensemble:
+    python -m tdscha_amd.gamma [--L 6] [--configs 20000] [--precision bf16x3] [--reps 10]
+    Two chains (L q and L^T p, q and p drawn in their sectors) per batched application; the
+    image sum over the representatives and the projection are timed apart with events."""
+    import argparse
+    import json
+    import time
+
+    import torch
+
+    from . import synthetic as sy
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--L", type=int, default=6)
+    ap.add_argument("--configs", type=int, default=20000)
+    ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args()
+    dev = torch.device("cuda", 0)
+    ens = sy.make_ensemble(a.L, a.configs, T=300.0, device=dev)
+    rots, irt = sy.space_group(ens.crystal)
+    sym = SymmetryBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    precs = ("fp64",) if a.precision == "fp64" else ("fp64", a.precision)
+    op = GammaSectorOperator(ens.X, ens.Y, ens.w, ens.rho, ens.T, sym, rots, irt, ens.pols,
+                             precisions=precs)
+    if a.precision == "bf16x3":
+        op.autotune(a.precision, 2)
+    n = op.n
+    g = torch.Generator().manual_seed(3)
+    flags = [0, L.OP_TRANSPOSE]
+    vs = []
+    for f in flags:
+        v = torch.randn(n + n * (n + 1), generator=g, dtype=torch.float64).to(dev)
+        v = op.proj.project(v, adjoint=op._adjoint(f))
+        vs.append(v / v.norm())
+    tmp = [torch.empty_like(v) for v in vs]
+    outs = [torch.empty_like(v) for v in vs]
+    st = torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_sum = t_proj = 0.0
+    for r in range(a.reps + 1):
+        ev[0].record(st)
+        op.ens.apply_full_L_batched(vs, flags, None, a.precision, outs=tmp)
+        ev[1].record(st)
+        for t, o, f in zip(tmp, outs, flags):
+            op.proj.project(t, op._adjoint(f), out=o)
+        ev[2].record(st)
+        ev[2].synchronize()
+        if r:                                  # the first application is the warm-up
+            t_sum += ev[0].elapsed_time(ev[1])
+            t_proj += ev[1].elapsed_time(ev[2])
+    ms = (t_sum + t_proj) / a.reps
+    print(json.dumps({"workload": "rock-salt %dx%dx%d, %d modes, %d configs, %d symmetries = "
+                                  "%d rotations x %d translations"
+                                  % (a.L, a.L, a.L, n, a.configs, ens.n_syms,
+                                     len(op.representatives), len(op.translations)),
+                      "precision": a.precision, "ms_per_batched_application": ms,
+                      "L_applications_per_s": 2e3 / ms,
+                      "image_sum_ms": t_sum / a.reps, "projection_ms": t_proj / a.reps,
+                      "tuning": op.tuning()}), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
