@@ -79,3 +79,25 @@ def test_reduction_needs_the_invariant_sector(rock_salt):
     b = orc.translation_average_psi(orc.apply_full_L(rep, p, transpose=True), ens.pols,
                                     irt[tr])
     assert rel(b, a) > 1e-6
+
+
+def test_orbit_fold_is_the_translation_average(rock_salt):
+    """The folded average (orbit x orbit x relative translation, the projector's fast form)
+    equals the average over the translations' atom permutations, done here by brute force
+    on the host with numpy; a non-free action has no fold."""
+    from tdscha_amd.gamma import _orbit_fold
+    ens, rots, irt, reps, tr, _, _ = rock_salt
+    it = irt[tr]
+    nt, nat = it.shape
+    s, idx, nb = _orbit_fold(it)
+    assert nb * nt == nat and np.all(np.bincount(s) == nt)
+    rng = np.random.default_rng(7)
+    C = rng.standard_normal((nat, nat, 3, 3))
+    brute = np.mean([C[np.ix_(p, p)] for p in it], axis=0)
+    G = np.zeros((nb * nb * nt, 3, 3))
+    np.add.at(G, idx.ravel(), C.reshape(nat * nat, 3, 3))
+    fold = (G[idx.ravel()] / nt).reshape(nat, nat, 3, 3)
+    assert np.abs(fold - brute).max() < 1e-14
+    fixed = it.copy()
+    fixed[1] = np.arange(nat)          # a second identity: not a free action
+    assert _orbit_fold(fixed) is None
