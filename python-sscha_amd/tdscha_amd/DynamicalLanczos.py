@@ -230,6 +230,11 @@ class Lanczos(object):
             if self.sc_symmetries is None or not use_symmetries:
                 raise ValueError("gamma_sector needs the space group: call "
                                  "prepare_symmetrization(symmetries=(rotations, irt)) first")
+            pols = np.asarray(self.pols)
+            if pols.ndim != 2 or pols.shape[1] != self.n_modes or pols.shape[0] % 3 or \
+                    pols.shape[0] == 0:
+                raise ValueError("gamma_sector needs the polarization vectors of the kept modes, "
+                                 "(3 nat, n_modes): set_ensemble(..., pols=...)")
             rot, irt = self.sc_symmetries
             self._dev_ens = GammaSectorOperator(self.X, self.Y, self.w, self.rho, self.T, sym,
                                                 rot, irt, self.pols, precisions=precs,

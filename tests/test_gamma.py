@@ -101,3 +101,22 @@ def test_orbit_fold_is_the_translation_average(rock_salt):
     fixed = it.copy()
     fixed[1] = np.arange(nat)          # a second identity: not a free action
     assert _orbit_fold(fixed) is None
+
+
+def test_gamma_sector_needs_the_space_group_and_the_polarization_vectors():
+    """Refused before any device work: no space group, or no polarization vectors."""
+    from tdscha_amd import DynamicalLanczos as DL
+    rng = np.random.default_rng(0)
+    n, N = 4, 10
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64)
+    lz.set_ensemble(rng.standard_normal((N, n)), rng.standard_normal((N, n)),
+                    np.linspace(1e-3, 2e-3, n), np.ones(N), 100.0)
+    lz.set_symmetries([np.ones((1, 1, 1))] * n, np.ones(n, dtype=np.intc),
+                      [np.array([i], dtype=np.intc) for i in range(n)],
+                      np.arange(n, dtype=np.intc))
+    lz.gamma_sector = True
+    with pytest.raises(ValueError, match="space group"):
+        lz.init()
+    lz.sc_symmetries = (np.eye(3)[None], np.arange(2)[None])
+    with pytest.raises(ValueError, match="polarization vectors"):
+        lz.init()
