@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define TDSCHA_ABI_VERSION 1
+#define TDSCHA_ABI_VERSION 2
 
 /* Arithmetic of the two ensemble contractions. */
 #define TDSCHA_PREC_FP64 0   /* FP64 MFMA, the plain path */
@@ -200,23 +200,32 @@ int tdscha_ensemble_force_grad(tdscha_ensemble *ens, int n_pairs, const int *fla
 /* ---- Device-resident Lanczos (the recurrence of Lanczos.run_FT, Py:5843-6372) ---------
  * n_chains starting vectors advance together: every step is one batched L application
  * (L q and L^T p of every chain, or L q only for the symmetric Wigner form) followed by the
- * FP64 recurrence on the device; a step is captured once as a hipGraph and replayed. There
- * is no re-orthogonalisation (the reference's default, n_rep_orth = 0). flags / shifts per
- * chain as for tdscha_apply_full_L (the TRANSPOSE bit is managed internally). mask (host,
- * length n + n (n + 1)) is the Wigner dot-product weight, needed when symmetric is 1.
- * keep_basis stores every q (and p) on the device. */
+ * FP64 recurrence on the device; a step is captured once as a hipGraph and replayed.
+ * full_gs = 0: no re-orthogonalisation (the reference's default, n_rep_orth = 0).
+ * full_gs = 1: every new pair is re-biorthogonalised against all stored pairs (two passes
+ * of classical two-sided Gram-Schmidt with the duals p_norm_j p_j, FP64, inside the same
+ * graph; implies keep_basis), as Lanczos.run_FT(full_gs=True) does on the host. The
+ * reference's own loop (Py:6245-6317) is not what this computes. flags / shifts per chain
+ * as for tdscha_apply_full_L (the TRANSPOSE bit is managed internally). mask (host, length
+ * n + n (n + 1)) is the Wigner dot-product weight, needed when symmetric is 1. keep_basis
+ * stores every q (and p) on the device. */
 typedef struct tdscha_lanczos tdscha_lanczos;
 int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precision,
                           int n_chains, const int *flags, const double *shifts, int symmetric,
-                          int max_steps, int keep_basis, const double *mask);
+                          int max_steps, int keep_basis, int full_gs, const double *mask);
 int tdscha_lanczos_destroy(tdscha_lanczos *lz);
 /* psi0[c]: device vectors; they are normalised as in run_FT (Py:5936-5945). */
 int tdscha_lanczos_start(tdscha_lanczos *lz, const double *const *psi0, void *stream);
+/* Enqueue n_steps steps; at most max_steps since the last start (error otherwise). */
 int tdscha_lanczos_run(tdscha_lanczos *lz, int n_steps, int use_graph, void *stream);
 /* Host copies of the first steps_done a / b / c / s_norm_(k+1) of a chain (any may be NULL);
  * synchronises the device. */
 int tdscha_lanczos_read(tdscha_lanczos *lz, int chain, double *a, double *b, double *c,
                         double *s_norm, int *steps_done);
+/* full_gs only: min(|q|, |p|) of each step's new pair after the projection, before its
+ * normalisation (below 1e-12 the new vector is linearly dependent on the stored ones and
+ * the run ends after the next a, as run_FT's next_converged does, Py:6283-6303). */
+int tdscha_lanczos_read_gs(tdscha_lanczos *lz, int chain, double *norms, int *steps_done);
 /* Device pointers to the stored basis of a chain ((max_steps + 1) x (n + n (n + 1))). */
 int tdscha_lanczos_basis(tdscha_lanczos *lz, int chain, const double **Q, const double **P);
 

@@ -560,13 +560,52 @@ class LanczosState:
     s_norm: list = field(default_factory=list)
 
 
+def biorthogonal_gs(new_q, new_p, Q, P, s_norm, c, dot, passes=2):
+    """Two-sided classical Gram-Schmidt, applied `passes` times, of a new Lanczos pair against
+    every stored pair (Q_j, P_j), followed by unit normalisation.
+
+    The duals of the stored left vectors are pt_j = P_j * s_norm_j / c_(j-1) (c_(-1) = 1):
+    the recurrence of Py:6059-6113 keeps pt_j . q_k = delta_jk. Each pass removes from q its
+    components along Q_j measured by the duals, and from p its components along the duals
+    measured by Q_j, all coefficients of a pass taken from the same vectors:
+        q <- q - sum_j (pt_j . q) Q_j,    p <- p - sum_j (Q_j . p) pt_j
+    Returns (q / |q|, p / |p|, |q|, |p|); the caller sets the new s_norm = c_k / (p . q) so
+    that the new dual is p * s_norm / c_k.
+
+    No counterpart in the reference: its loop (Py:6245-6317, the n_rep_orth branch of
+    run_FT below) subtracts (P_j . q) P_j from q and (Q_j . p) Q_j from p (the left vectors
+    along the right ones and vice versa, without the dual scale), normalises p by p . p and
+    never grows its ortho_p test; no fixture of the reference exercises it."""
+    q = new_q.copy()
+    p = new_p.copy()
+    pts = [P[j] * (s_norm[j] / (c[j - 1] if j >= 1 else 1.0)) for j in range(len(P))]
+    for _ in range(passes):
+        hq = [dot(pt, q) for pt in pts]
+        hp = [dot(Q[j], p) for j in range(len(Q))]
+        for j in range(len(Q)):
+            q = q - hq[j] * Q[j]
+            p = p - hp[j] * pts[j]
+    nq = np.sqrt(dot(q, q))
+    nP = np.sqrt(dot(p, p))
+    return q / nq, p / nP, nq, nP
+
+
 def run_FT(matvec, rmatvec, psi0, n_iter, run_simm=False, mask=None, n_rep_orth=0, n_ortho=10,
-           state: LanczosState | None = None):
+           state: LanczosState | None = None, full_gs=False):
     """Biconjugate / symmetric Lanczos of Py:5843-6372 (verbatim recurrence).
 
     matvec(q) = L q, rmatvec(p) = L^T p. Returns the LanczosState.
-    The Gram-Schmidt block follows Py:6245-6317 as written (including its
-    normalisation of p by p.p and its ortho_p accumulator).
+
+    full_gs=True re-biorthogonalises every new pair against all stored pairs with
+    ``biorthogonal_gs`` (two passes). Pinned by mathematics: the duals and the basis stay
+    biorthogonal (P~^T Q = I), the tridiagonal matrix of a complete Krylov space has the
+    eigenvalues of the operator, and the first steps equal the reference's own
+    unreorthogonalised run (where that run is still exact).
+
+    n_rep_orth > 0 follows the reference's Gram-Schmidt block Py:6245-6317 as written
+    (including its normalisation of p by p.p and its ortho_p accumulator). That branch is a
+    documented quirk of the reference and is PARITY UNPINNED: no reference fixture uses it,
+    and its p normalisation breaks the recurrence's own biorthogonality.
     """
     st = state if state is not None else LanczosState()
     dot = (lambda u, v: u.dot(v * mask)) if run_simm else (lambda u, v: u.dot(v))
@@ -602,6 +641,12 @@ def run_FT(matvec, rmatvec, psi0, n_iter, run_simm=False, mask=None, n_rep_orth=
             break
         new_q = rk / b_coeff
         new_p = sk_tilde.copy()
+        if full_gs:
+            new_q, new_p, nq, nP = biorthogonal_gs(new_q, new_p, st.Q, st.P, st.s_norm, st.c,
+                                                   dot)
+            if nq < EPSILON or nP < EPSILON:
+                next_converged = True
+            s_norm = c_coeff / dot(new_p, new_q)
         for _k in range(n_rep_orth):
             ortho_q = 0.0
             ortho_p = 0.0

@@ -1,7 +1,7 @@
 // Device-resident biconjugate / symmetric Lanczos on several chains, one hipGraph per step.
 //
-// The recurrence is the reference's run_FT (Modules/DynamicalLanczos.py:5843-6372, without
-// re-orthogonalisation): with p_norm_k = s_norm_k / c_(k-1),
+// The recurrence is the reference's run_FT (Modules/DynamicalLanczos.py:5843-6372): with
+// p_norm_k = s_norm_k / c_(k-1),
 //   a_k   = p_norm_k <p_k, L q_k>
 //   r_k   = L q_k - a_k q_k - c_(k-1) q_(k-1)
 //   s_k   = L^T p_k - a_k p_k - b_(k-1) (p_norm_(k-1) / p_norm_k) p_(k-1)
@@ -12,6 +12,15 @@
 // each step appends a_k, b_k, c_k, s_norm_(k+1) to device arrays at a device step counter,
 // so one captured graph serves every step. The host truncates at the first |b| or |c|
 // below 1e-12 (the reference's stopping rule, Py:6144-6153).
+//
+// Optional full re-biorthogonalisation (full_gs): after q_(k+1) = r_k / b_k and
+// p_(k+1) = s_k / |s_k| the new pair is projected, in two passes of classical two-sided
+// Gram-Schmidt, against every stored pair j = 0..k with the duals pt_j = p_norm_j p_j:
+//   q <- q - sum_j (pt_j . q) q_j,   p <- p - sum_j (q_j . p) pt_j
+// then normalised, and p_norm_(k+1) = 1 / (p . q) (s_norm_(k+1) = c_k p_norm_(k+1)), as
+// run_FT(full_gs=True) does on the host. The number of stored pairs is read from the device
+// step counter, so the projection sits inside the same captured graph; every coefficient is
+// a fixed-order FP64 sum (partials over fixed vector chunks, then over the chunks in order).
 #include <string>
 #include <vector>
 
@@ -22,6 +31,7 @@ namespace tdscha {
 namespace {
 
 constexpr int kDotBlocks = 256;  // fixed partial count: reductions have a fixed order
+constexpr int kGsBlocks = 128;   // vector chunks of the Gram-Schmidt projections
 
 // state per chain: [0] p_norm, [1] old p_norm, [2] c_(k-1), [3] b_(k-1), [4] a_k
 enum { ST_PN = 0, ST_OPN, ST_CL, ST_BL, ST_A, ST_N };
@@ -170,11 +180,166 @@ __global__ __launch_bounds__(256) void k_shift(const ChainVecs *vecs, const doub
   }
 }
 
+// ---- full re-biorthogonalisation ------------------------------------------------------
+// Stored pairs of chain c: Q_j = basisQ[c][j], P_j = basisP[c][j] (= Q_j in the symmetric
+// form), j < K = min(*step, max_steps + 1); the new pair is in v.q, v.p.
+
+// partial[c][j][blk] = (P_j . q, Q_j . p) over chunk blk of the vector (mask-weighted)
+__global__ __launch_bounds__(256) void k_gs_dots(const ChainVecs *vecs, const double *basisQ,
+                                                 const double *basisP,
+                                                 const double *__restrict__ mask, long n,
+                                                 int symmetric, const int *step, int max_steps,
+                                                 double *partial) {
+  __shared__ double sh[4];
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  const int K = min(*step, max_steps + 1);
+  const size_t cb = (size_t)c * (max_steps + 1);
+  const long chunk = (n + kGsBlocks - 1) / kGsBlocks;
+  const long i0 = (long)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  for (int j = 0; j < K; ++j) {
+    const double *Qj = basisQ + (cb + j) * n;
+    const double *Pj = symmetric ? Qj : basisP + (cb + j) * n;
+    double hq = 0.0, hp = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const double m = mask ? mask[i] : 1.0;
+      hq = fma(Pj[i], v.q[i] * m, hq);
+      if (!symmetric) hp = fma(Qj[i], v.p[i] * m, hp);
+    }
+    hq = block_sum(hq, sh);
+    if (!symmetric) hp = block_sum(hp, sh);
+    if (threadIdx.x == 0) {
+      double *pc = partial + ((size_t)c * (max_steps + 1) + j) * 2 * kGsBlocks;
+      pc[blockIdx.x] = hq;
+      pc[kGsBlocks + blockIdx.x] = hp;
+    }
+  }
+}
+
+// coef[c][j] = (p_norm_j * sum_blk P_j . q, sum_blk Q_j . p), the chunks added in order
+__global__ __launch_bounds__(256) void k_gs_coeffs(const double *partial, const double *pn_all,
+                                                   const int *step, int max_steps, double *coef) {
+  const int c = blockIdx.x;
+  const int K = min(*step, max_steps + 1);
+  for (int j = threadIdx.x; j < K; j += blockDim.x) {
+    const size_t cj = (size_t)c * (max_steps + 1) + j;
+    const double *pc = partial + cj * 2 * kGsBlocks;
+    double hq = 0.0, hp = 0.0;
+    for (int b = 0; b < kGsBlocks; ++b) {
+      hq += pc[b];
+      hp += pc[kGsBlocks + b];
+    }
+    coef[2 * cj] = pn_all[cj] * hq;
+    coef[2 * cj + 1] = hp;
+  }
+}
+
+// q -= sum_j coef_q[j] Q_j ;  p -= sum_j coef_p[j] p_norm_j P_j  (j in order)
+__global__ __launch_bounds__(256) void k_gs_update(const ChainVecs *vecs, const double *basisQ,
+                                                   const double *basisP, const double *coef,
+                                                   const double *pn_all, long n, int symmetric,
+                                                   const int *step, int max_steps) {
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  const int K = min(*step, max_steps + 1);
+  const size_t cb = (size_t)c * (max_steps + 1);
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double q = v.q[i];
+    double p = symmetric ? 0.0 : v.p[i];
+    for (int j = 0; j < K; ++j) {
+      const double *Qj = basisQ + (cb + j) * n;
+      q = fma(-coef[2 * (cb + j)], Qj[i], q);
+      if (!symmetric) {
+        const double *Pj = basisP + (cb + j) * n;
+        p = fma(-coef[2 * (cb + j) + 1] * pn_all[cb + j], Pj[i], p);
+      }
+    }
+    v.q[i] = q;
+    if (!symmetric) v.p[i] = p;
+  }
+}
+
+// partial sums of q.q, p.p, p.q (mask-weighted) after the projection
+__global__ __launch_bounds__(256) void k_gs_norms(const ChainVecs *vecs,
+                                                  const double *__restrict__ mask, long n,
+                                                  int symmetric, double *partial) {
+  __shared__ double sh[4];
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  double qq = 0.0, pp = 0.0, pq = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double m = mask ? mask[i] : 1.0;
+    const double q = v.q[i];
+    const double p = symmetric ? q : v.p[i];
+    qq = fma(q, q * m, qq);
+    pp = fma(p, p * m, pp);
+    pq = fma(p, q * m, pq);
+  }
+  qq = block_sum(qq, sh);
+  pp = block_sum(pp, sh);
+  pq = block_sum(pq, sh);
+  if (threadIdx.x == 0) {
+    partial[(c * 3 + 0) * kDotBlocks + blockIdx.x] = qq;
+    partial[(c * 3 + 1) * kDotBlocks + blockIdx.x] = pp;
+    partial[(c * 3 + 2) * kDotBlocks + blockIdx.x] = pq;
+  }
+}
+
+// |q|, |p| and the new dual scale p_norm_(k+1) = |q| |p| / (p . q) of the normalised pair;
+// s_norm_(k+1) = c_k p_norm_(k+1) replaces the value k_finish_bc recorded. gsn[k] holds
+// min(|q|, |p|) (a linearly dependent new vector: the host ends the run there, as run_FT's
+// next_converged does).
+__global__ __launch_bounds__(256) void k_gs_finish(const double *partial, double *state,
+                                                   double *pn_all, double *s_all, double *gsn,
+                                                   double *scal, const int *step, int max_steps) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  const double qq = sum_partials(partial + (c * 3 + 0) * kDotBlocks, sh);
+  const double pp = sum_partials(partial + (c * 3 + 1) * kDotBlocks, sh);
+  const double pq = sum_partials(partial + (c * 3 + 2) * kDotBlocks, sh);
+  if (threadIdx.x == 0) {
+    double *st = state + c * ST_N;
+    const double nq = sqrt(qq), np_ = sqrt(pp);
+    const double pn = (nq * np_) / pq;
+    st[ST_PN] = pn;
+    const int k1 = *step;  // index of the new pair
+    if (k1 <= max_steps) pn_all[(size_t)c * (max_steps + 1) + k1] = pn;
+    if (k1 - 1 < max_steps) {
+      s_all[c * max_steps + k1 - 1] = st[ST_CL] * pn;
+      gsn[c * max_steps + k1 - 1] = fmin(nq, np_);
+    }
+    scal[2 * c] = 1.0 / nq;
+    scal[2 * c + 1] = 1.0 / np_;
+  }
+}
+
+// q /= |q|, p /= |p|, stored as pair k + 1 of the basis
+__global__ __launch_bounds__(256) void k_gs_scale(const ChainVecs *vecs, const double *scal, long n,
+                                                  int symmetric, double *basisQ, double *basisP,
+                                                  const int *step, int max_steps) {
+  const int c = blockIdx.y;
+  const ChainVecs v = vecs[c];
+  const double iq = scal[2 * c], ip = scal[2 * c + 1];
+  const int k = *step;
+  double *bq = k <= max_steps ? basisQ + ((size_t)c * (max_steps + 1) + k) * n : nullptr;
+  double *bp = (basisP && k <= max_steps) ? basisP + ((size_t)c * (max_steps + 1) + k) * n : nullptr;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double q = v.q[i] * iq;
+    v.q[i] = q;
+    if (bq) bq[i] = q;
+    if (!symmetric) {
+      const double p = v.p[i] * ip;
+      v.p[i] = p;
+      if (bp) bp[i] = p;
+    }
+  }
+}
+
 // q = p = psi / sqrt(psi . psi (* mask)), previous vectors zero, state reset
 __global__ __launch_bounds__(256) void k_start(const ChainVecs *vecs, const double *const *psi,
                                                const double *partial, long n, int symmetric,
                                                double *state, double *basisQ, double *basisP,
-                                               int max_steps) {
+                                               double *pn_all, int max_steps) {
   __shared__ double sh[4];
   __shared__ double inv;
   const int c = blockIdx.y;
@@ -197,6 +362,7 @@ __global__ __launch_bounds__(256) void k_start(const ChainVecs *vecs, const doub
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     double *st = state + c * ST_N;
+    if (pn_all) pn_all[(size_t)c * (max_steps + 1)] = 1.0;
     st[ST_PN] = 1.0;   // s_norm_0 / c_(-1) = 1
     st[ST_OPN] = 0.0;
     st[ST_CL] = 0.0;
@@ -224,11 +390,18 @@ struct tdscha_lanczos {
   double *vec = nullptr;         // 6 vectors per chain
   ChainVecs *dvecs = nullptr;
   const double **dpsi_in = nullptr, **dpsi_p = nullptr;  // pointer tables for the dots
+  const double **dpsi0 = nullptr;                          // starting vectors
   double *mask = nullptr;
   double *partial = nullptr, *state = nullptr, *scal = nullptr;
   double *a_all = nullptr, *b_all = nullptr, *c_all = nullptr, *s_all = nullptr;
   double *basisQ = nullptr, *basisP = nullptr;
   int *step = nullptr;
+  // full re-biorthogonalisation: dual scales p_norm_j [C x (max_steps + 1)], projection
+  // partials [C x (max_steps + 1) x 2 x kGsBlocks] and coefficients [C x (max_steps + 1) x 2],
+  // norm of each new pair after the projection [C x max_steps]
+  int full_gs = 0;
+  double *pn_all = nullptr, *gs_part = nullptr, *gs_coef = nullptr, *gsn = nullptr;
+  long host_steps = 0;   // steps enqueued since the last start
   std::vector<const double *> apply_in;
   std::vector<double *> apply_out;
   std::vector<int> apply_flags;
@@ -271,7 +444,26 @@ int enqueue_step(tdscha_lanczos *z, hipStream_t st) {
                      z->c_all, z->s_all, z->scal, z->step, C, z->max_steps);
   hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, st, z->step);
   hipLaunchKernelGGL(k_shift, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, z->scal, n,
-                     z->symmetric, z->basisQ, z->basisP, z->step, z->max_steps);
+                     z->symmetric, z->full_gs ? (double *)nullptr : z->basisQ,
+                     z->full_gs ? (double *)nullptr : z->basisP, z->step, z->max_steps);
+  if (z->full_gs) {
+    const double *m = z->symmetric ? z->mask : (const double *)nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+      hipLaunchKernelGGL(k_gs_dots, dim3(kGsBlocks, C), dim3(256), 0, st, z->dvecs, z->basisQ,
+                         z->basisP, m, n, z->symmetric, z->step, z->max_steps, z->gs_part);
+      hipLaunchKernelGGL(k_gs_coeffs, dim3(C), dim3(256), 0, st, z->gs_part, z->pn_all, z->step,
+                         z->max_steps, z->gs_coef);
+      hipLaunchKernelGGL(k_gs_update, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, z->basisQ,
+                         z->basisP, z->gs_coef, z->pn_all, n, z->symmetric, z->step,
+                         z->max_steps);
+    }
+    hipLaunchKernelGGL(k_gs_norms, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, m, n,
+                       z->symmetric, z->partial);
+    hipLaunchKernelGGL(k_gs_finish, dim3(C), dim3(256), 0, st, z->partial, z->state, z->pn_all,
+                       z->s_all, z->gsn, z->scal, z->step, z->max_steps);
+    hipLaunchKernelGGL(k_gs_scale, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs, z->scal, n,
+                       z->symmetric, z->basisQ, z->basisP, z->step, z->max_steps);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("Lanczos step launch failed: ") + hipGetErrorString(e));
@@ -286,7 +478,7 @@ extern "C" {
 
 int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precision, int n_chains,
                           const int *flags, const double *shifts, int symmetric, int max_steps,
-                          int keep_basis, const double *mask) {
+                          int keep_basis, int full_gs, const double *mask) {
   if (!out || !ens || !flags || !shifts || n_chains < 1 || max_steps < 1) {
     set_error("bad argument to tdscha_lanczos_create");
     return 1;
@@ -311,7 +503,8 @@ int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precis
   z->nc = n_chains;
   z->symmetric = symmetric;
   z->max_steps = max_steps;
-  z->keep_basis = keep_basis;
+  z->full_gs = full_gs ? 1 : 0;
+  z->keep_basis = (keep_basis || full_gs) ? 1 : 0;  // the projection reads the stored pairs
   z->n_psi = n + n * (n + 1);
   z->flags.assign(flags, flags + n_chains);
   z->shifts.assign(shifts, shifts + n_chains);
@@ -323,6 +516,7 @@ int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precis
   rc |= lz_alloc(z, (void **)&z->dvecs, sizeof(ChainVecs) * C);
   rc |= lz_alloc(z, (void **)&z->dpsi_in, sizeof(double *) * 2 * C);
   rc |= lz_alloc(z, (void **)&z->dpsi_p, sizeof(double *) * C);
+  rc |= lz_alloc(z, (void **)&z->dpsi0, sizeof(double *) * C);
   rc |= lz_alloc(z, (void **)&z->partial, sizeof(double) * 3 * C * kDotBlocks);
   rc |= lz_alloc(z, (void **)&z->state, sizeof(double) * ST_N * C);
   rc |= lz_alloc(z, (void **)&z->scal, sizeof(double) * 2 * C);
@@ -332,9 +526,16 @@ int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precis
   rc |= lz_alloc(z, (void **)&z->s_all, sizeof(double) * C * max_steps);
   rc |= lz_alloc(z, (void **)&z->step, sizeof(int));
   if (symmetric) rc |= lz_alloc(z, (void **)&z->mask, sizeof(double) * N);
-  if (keep_basis) {
+  if (z->keep_basis) {
     rc |= lz_alloc(z, (void **)&z->basisQ, sizeof(double) * C * (max_steps + 1) * N);
     if (!symmetric) rc |= lz_alloc(z, (void **)&z->basisP, sizeof(double) * C * (max_steps + 1) * N);
+  }
+  if (z->full_gs) {
+    const size_t nv = (size_t)C * (max_steps + 1);
+    rc |= lz_alloc(z, (void **)&z->pn_all, sizeof(double) * nv);
+    rc |= lz_alloc(z, (void **)&z->gs_part, sizeof(double) * nv * 2 * kGsBlocks);
+    rc |= lz_alloc(z, (void **)&z->gs_coef, sizeof(double) * nv * 2);
+    rc |= lz_alloc(z, (void **)&z->gsn, sizeof(double) * C * max_steps);
   }
   if (rc) { tdscha_lanczos_destroy(z); return rc; }
   std::vector<ChainVecs> hv(C);
@@ -388,13 +589,10 @@ int tdscha_lanczos_start(tdscha_lanczos *z, const double *const *psi0, void *str
   if (!z || !psi0) { set_error("NULL argument"); return 1; }
   hipStream_t st = (hipStream_t)stream;
   const int C = z->nc;
-  const double **dpsi = nullptr;
   // the starting vectors are read through a device pointer table
   std::vector<const double *> hp(psi0, psi0 + C);
-  const double **tab = nullptr;
-  if (hipMalloc(&tab, sizeof(double *) * C) != hipSuccess) { set_error("hipMalloc failed"); return 11; }
-  z->allocs.push_back(tab);
-  dpsi = tab;
+  const double **dpsi = z->dpsi0;
+  z->host_steps = 0;
   if (hipMemcpy(dpsi, hp.data(), sizeof(double *) * C, hipMemcpyHostToDevice) != hipSuccess) {
     set_error("pointer table copy failed");
     return 10;
@@ -404,7 +602,9 @@ int tdscha_lanczos_start(tdscha_lanczos *z, const double *const *psi0, void *str
                      z->symmetric ? z->mask : (const double *)nullptr, z->n_psi, z->partial);
   hipLaunchKernelGGL(k_start, dim3(kDotBlocks, C), dim3(256), 0, st, z->dvecs,
                      (const double *const *)dpsi, z->partial, z->n_psi, z->symmetric, z->state,
-                     z->basisQ, z->basisP, z->max_steps);
+                     z->basisQ, z->basisP, z->pn_all, z->max_steps);
+  if (z->gsn)
+    (void)hipMemsetAsync(z->gsn, 0x7f, sizeof(double) * C * z->max_steps, st);  // large, finite
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, z->step);
   hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
@@ -416,6 +616,13 @@ int tdscha_lanczos_start(tdscha_lanczos *z, const double *const *psi0, void *str
 
 int tdscha_lanczos_run(tdscha_lanczos *z, int n_steps, int use_graph, void *stream) {
   if (!z || n_steps < 0) { set_error("bad argument"); return 1; }
+  if (z->host_steps + n_steps > z->max_steps) {
+    set_error("more steps than max_steps since the last start (" +
+              std::to_string(z->host_steps) + " + " + std::to_string(n_steps) + " > " +
+              std::to_string(z->max_steps) + ")");
+    return 1;
+  }
+  z->host_steps += n_steps;
   hipStream_t st = (hipStream_t)stream;
   if (!use_graph) {
     for (int i = 0; i < n_steps; ++i) {
@@ -430,15 +637,16 @@ int tdscha_lanczos_run(tdscha_lanczos *z, int n_steps, int use_graph, void *stre
       set_error("stream creation failed");
       return 10;
     }
-    int prof = tdscha_profile_enable(0);  // no event records inside a captured graph
-    (void)prof;
+    const bool prof = profile_swap(false);  // no event records inside a captured graph
     hipError_t e = hipStreamBeginCapture(z->graph_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
+      profile_swap(prof);
       set_error(std::string("stream capture failed: ") + hipGetErrorString(e));
       return 10;
     }
     int rc = enqueue_step(z, z->graph_stream);
     e = hipStreamEndCapture(z->graph_stream, &z->graph);
+    profile_swap(prof);
     if (rc) return rc;
     if (e != hipSuccess) {
       set_error(std::string("end capture failed: ") + hipGetErrorString(e));
@@ -521,9 +729,12 @@ int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *cons
     if (e == hipSuccess) g->joins.push_back(j);
   }
   if (e != hipSuccess) return fail("stream / event creation failed", e);
-  tdscha_profile_enable(0);  // no timing events inside a captured graph
+  const bool prof = profile_swap(false);  // no timing events inside a captured graph
   e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return fail("stream capture failed", e);
+  if (e != hipSuccess) {
+    profile_swap(prof);
+    return fail("stream capture failed", e);
+  }
   int rc = 0;
   e = hipEventRecord(g->fork, g->cap);
   for (int i = 0; i < n_members && e == hipSuccess && !rc; ++i) {
@@ -534,6 +745,7 @@ int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *cons
   }
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(g->cap, &graph);
+  profile_swap(prof);
   g->graph = graph;
   if (rc) { tdscha_lanczos_group_destroy(g); return rc; }
   if (e != hipSuccess) return fail("capturing the group step failed", e);
@@ -546,6 +758,12 @@ int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *cons
 
 int tdscha_lanczos_group_run(tdscha_lanczos_group *g, int n_steps, void *stream) {
   if (!g || n_steps < 0) { set_error("bad argument"); return 1; }
+  for (tdscha_lanczos *m : g->members)
+    if (m->host_steps + n_steps > m->max_steps) {
+      set_error("a member of the group would run past its max_steps");
+      return 1;
+    }
+  for (tdscha_lanczos *m : g->members) m->host_steps += n_steps;
   for (int i = 0; i < n_steps; ++i) {
     hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
     if (e != hipSuccess) {
@@ -570,6 +788,27 @@ int tdscha_lanczos_read(tdscha_lanczos *z, int chain, double *a, double *b, doub
   if (e == hipSuccess && s_norm) e = hipMemcpy(s_norm, z->s_all + off, bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) {
     set_error(std::string("reading the coefficients failed: ") + hipGetErrorString(e));
+    return 10;
+  }
+  *steps_done = k;
+  return 0;
+}
+
+int tdscha_lanczos_read_gs(tdscha_lanczos *z, int chain, double *norms, int *steps_done) {
+  if (!z || chain < 0 || chain >= z->nc || !norms || !steps_done) {
+    set_error("bad argument");
+    return 1;
+  }
+  if (!z->full_gs) { set_error("the Lanczos was created without full_gs"); return 1; }
+  int k = 0;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(&k, z->step, sizeof(int), hipMemcpyDeviceToHost);
+  if (k > z->max_steps) k = z->max_steps;
+  if (e == hipSuccess)
+    e = hipMemcpy(norms, z->gsn + (size_t)chain * z->max_steps, sizeof(double) * k,
+                  hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_error(std::string("reading the projection norms failed: ") + hipGetErrorString(e));
     return 10;
   }
   *steps_done = k;

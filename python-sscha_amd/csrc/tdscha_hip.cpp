@@ -316,6 +316,18 @@ int tdscha_profile_enable(int on) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace tdscha {
+bool profile_swap(bool on) {
+  const bool was = g_prof.on;
+  g_prof.on = on;
+  return was;
+}
+}  // namespace tdscha
+
+extern "C" {
+
 int tdscha_profile_reset(void) {
   for (auto &r : g_prof.recs) {
     g_prof.pool.push_back(r.a);

@@ -136,6 +136,9 @@ struct Ensemble {
 };
 
 void set_error(const std::string &msg);
+// turn the kernel-group event timing on / off (no event records may enter a captured
+// graph); returns the previous setting
+bool profile_swap(bool on);
 
 // ---- kernels / launchers (rotate.hip) ----
 void launch_prep_inputs(const Ensemble &e, const double *R1, const double *Y1, hipStream_t st);

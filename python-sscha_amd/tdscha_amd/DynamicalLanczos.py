@@ -20,9 +20,10 @@ Deviations from the reference, on purpose:
     (set_ensemble_from_configs, projected on the device); CellConstructor / sscha objects
     are not required.
   * Errors raise ValueError / RuntimeError instead of exiting the interpreter.
-  * run_FT(full_gs=True) adds a two-sided Gram-Schmidt against every stored vector with
-    the biorthogonal duals p_j * s_norm_j / c_(j-1); n_rep_orth keeps the reference's
-    own loop (Py:6245-6317) verbatim.
+  * run_FT(full_gs=True) adds a two-sided classical Gram-Schmidt (two passes) against every
+    stored vector with the biorthogonal duals p_j * s_norm_j / c_(j-1), on the Python-driven
+    and on the device-resident recurrence; n_rep_orth keeps the reference's own loop
+    (Py:6245-6317) verbatim, quirks included.
 """
 import os
 import sys
@@ -692,15 +693,16 @@ class Lanczos(object):
         """Biconjugate (or Wigner symmetric) Lanczos, Py:5843-6372, Krylov vectors on device.
 
         device_loop=True runs the whole recurrence on the GPU (tdscha_lanczos_*, one hipGraph
-        per step when graph=True); it needs a fresh start, n_rep_orth = 0 and no full_gs.
+        per step when graph=True), with or without full_gs; it needs a fresh start and
+        n_rep_orth = 0.
         """
         import torch
         self.verbose = verbose
         ens = self.device_ensemble
         if device_loop:
-            if len(self.a_coeffs) or n_rep_orth or full_gs:
-                raise ValueError("device_loop needs a fresh start, n_rep_orth=0, full_gs=False")
-            return self._run_device_loop(n_iter, run_simm, graph, save_dir, prefix)
+            if len(self.a_coeffs) or n_rep_orth:
+                raise ValueError("device_loop needs a fresh start and n_rep_orth=0")
+            return self._run_device_loop(n_iter, run_simm, graph, save_dir, prefix, full_gs)
         dev = ens.device
         psi_host = np.asarray(self.psi, dtype=TYPE_DP)
         psi_norm = np.sum(psi_host ** 2)
@@ -708,6 +710,8 @@ class Lanczos(object):
             raise ValueError("Error, you must initialize a perturbation to start the Lanczos.")
         if run_simm and not self.use_wigner:
             raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
+        if full_gs and optimized:
+            raise ValueError("full_gs needs every stored vector; optimized=True drops them")
         if save_dir is not None and not os.path.exists(save_dir):
             os.makedirs(save_dir)
 
@@ -740,6 +744,23 @@ class Lanczos(object):
         if not (len(Q) == len(P) == len(s_norm_l)) or len(b_l) != len(c_l):
             raise ValueError("Something wrong when restoring the Lanczos.")
 
+        if self.gamma_sector:
+            # the starting vector was checked above and L keeps the sector
+            with ens.unchecked():
+                return self._run_FT_steps(Q, P, s_norm_l, a_l, b_l, c_l, i_step, n_iter,
+                                          save_dir, save_each, verbose, n_rep_orth, n_ortho,
+                                          flush_output, prefix, run_simm, optimized, full_gs,
+                                          dot, mask, dev)
+        return self._run_FT_steps(Q, P, s_norm_l, a_l, b_l, c_l, i_step, n_iter, save_dir,
+                                  save_each, verbose, n_rep_orth, n_ortho, flush_output, prefix,
+                                  run_simm, optimized, full_gs, dot, mask, dev)
+
+    def _run_FT_steps(self, Q, P, s_norm_l, a_l, b_l, c_l, i_step, n_iter, save_dir, save_each,
+                      verbose, n_rep_orth, n_ortho, flush_output, prefix, run_simm, optimized,
+                      full_gs, dot, mask, dev):
+        """The step loop of run_FT (Py:6007-6365) on device vectors."""
+        import torch
+        ens = self.device_ensemble
         psi_q, psi_p = Q[-1], P[-1]
         work_q = torch.empty_like(psi_q)
         work_p = torch.empty_like(psi_q)
@@ -790,13 +811,21 @@ class Lanczos(object):
             new_p = sk_tilde.clone()
 
             if full_gs:
-                # two-sided Gram-Schmidt against every stored pair, twice
+                # two-sided classical Gram-Schmidt against every stored pair, two passes:
+                # q -= sum_j (pt_j . q) Q_j, p -= sum_j (Q_j . p) pt_j with the duals
+                # pt_j = P_j s_norm_j / c_(j-1); every coefficient of a pass comes from the
+                # same vectors (the form the device-resident recurrence uses)
+                Qm = torch.stack(Q)
+                pns = torch.as_tensor([s_norm_l[j] / (c_l[j - 1] if j >= 1 else 1.0)
+                                       for j in range(len(P))], dtype=Qm.dtype, device=dev)
+                Pt = torch.stack(P) * pns[:, None]
+                wq = mask if run_simm else None
                 for _rep in range(2):
-                    for j in range(len(P)):
-                        pn = s_norm_l[j] / (c_l[j - 1] if j >= 1 else 1.0)
-                        pd = P[j] * pn
-                        new_q -= dot(pd, new_q) * Q[j]
-                        new_p -= dot(Q[j], new_p) * pd
+                    hq = Pt @ (new_q * wq if wq is not None else new_q)
+                    hp = Qm @ (new_p * wq if wq is not None else new_p)
+                    new_q -= hq @ Qm
+                    new_p -= hp @ Pt
+                del Qm, Pt
                 normq = float(torch.sqrt(dot(new_q, new_q)))
                 normp = float(torch.sqrt(dot(new_p, new_p)))
                 if normq < __EPSILON__ or normp < __EPSILON__:
@@ -855,7 +884,7 @@ class Lanczos(object):
         self.basis_P = [p.cpu().numpy() for p in P]
         return self
 
-    def _run_device_loop(self, n_iter, run_simm, graph, save_dir, prefix):
+    def _run_device_loop(self, n_iter, run_simm, graph, save_dir, prefix, full_gs=False):
         from .lanczos_device import DeviceLanczos
         ens = self.device_ensemble
         if self.gamma_sector:
@@ -870,7 +899,7 @@ class Lanczos(object):
         fl = self._flags(False) & ~L.OP_TRANSPOSE
         dl = DeviceLanczos(ens, 1, self.precision, [fl], [float(self.shift_value)],
                            symmetric=run_simm, max_steps=n_iter, keep_basis=True,
-                           mask=self.mask_dot_wigner() if run_simm else None)
+                           mask=self.mask_dot_wigner() if run_simm else None, full_gs=full_gs)
         dl.start([psi])
         dl.run(n_iter, graph=graph)
         a, b, c, s = dl.coefficients(0)

@@ -56,7 +56,8 @@ SIGNATURES = {
         _vp]),
     "tdscha_lanczos_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
-        ctypes.c_int, _dp]),
+        ctypes.c_int, ctypes.c_int, _dp]),
+    "tdscha_lanczos_read_gs": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _ip]),
     "tdscha_lanczos_destroy": (ctypes.c_int, [_vp]),
     "tdscha_lanczos_start": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "tdscha_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),

@@ -121,6 +121,20 @@ class DeviceEnsemble:
             raise ValueError("expected shape %s, got %s" % (shape, tuple(a.shape)))
         return a
 
+    def _check_out(self, o, shape, what="out"):
+        """A caller's output buffer: the kernels write its whole extent through a raw device
+        pointer, so anything but a contiguous FP64 tensor of the expected shape on this
+        ensemble's device is refused before a launch."""
+        t = self.torch
+        if not isinstance(o, t.Tensor) or o.dtype != t.float64 or o.device != self.device or \
+                not o.is_contiguous() or tuple(o.shape) != tuple(shape):
+            desc = ("%s %s %s%s" % (o.dtype, tuple(o.shape), o.device,
+                                    "" if o.is_contiguous() else " non-contiguous")
+                    if isinstance(o, t.Tensor) else type(o).__name__)
+            raise ValueError("%s must be a contiguous float64 tensor of shape %s on %s, got %s"
+                             % (what, tuple(shape), self.device, desc))
+        return o
+
     def apply(self, R1, Y1, terms=L.TERM_F | L.TERM_D2V_R | L.TERM_D2V_Y, precision="fp64",
               f=None, d2v=None, d2v_accumulate=False):
         """Perturbed averages (f_av, d2v) as device tensors (GetPerturbAverageSym contract)."""
@@ -130,8 +144,12 @@ class DeviceEnsemble:
         Y1 = self._dev(Y1, (n, n))
         if f is None:
             f = t.empty(n, dtype=t.float64, device=self.device)
+        else:
+            self._check_out(f, (n,), "f")
         if d2v is None:
             d2v = t.zeros((n, n), dtype=t.float64, device=self.device)
+        else:
+            self._check_out(d2v, (n, n), "d2v")
         ptr = lambda x: ctypes.c_void_p(x.data_ptr()) if x is not None else None
         rc = L.lib().tdscha_ensemble_apply(self._h, L.PRECISIONS[precision], int(terms), ptr(R1),
                                            ptr(Y1), ptr(f), ptr(d2v), int(bool(d2v_accumulate)),
@@ -146,6 +164,10 @@ class DeviceEnsemble:
         psi = self._dev(psi, (n + n * (n + 1),))
         if out is None:
             out = t.empty_like(psi)
+        else:
+            self._check_out(out, psi.shape)
+            if out.data_ptr() == psi.data_ptr():
+                raise ValueError("out must not alias psi")
         rc = L.lib().tdscha_apply_full_L(self._h, L.PRECISIONS[precision], int(flags), float(shift),
                                          ctypes.c_void_p(psi.data_ptr()),
                                          ctypes.c_void_p(out.data_ptr()),
@@ -162,6 +184,14 @@ class DeviceEnsemble:
         psis = [self._dev(p, (n + n * (n + 1),)) for p in psis]
         if outs is None:
             outs = [t.empty_like(p) for p in psis]
+        else:
+            if len(outs) != C:
+                raise ValueError("expected %d output tensors, got %d" % (C, len(outs)))
+            ins = {p.data_ptr() for p in psis}
+            for o in outs:
+                self._check_out(o, (n + n * (n + 1),), "each of outs")
+                if o.data_ptr() in ins:
+                    raise ValueError("an output must not alias an input")
         flags = [0] * C if flags is None else list(flags)
         shifts = [0.0] * C if shifts is None else list(shifts)
         fl = (ctypes.c_int * C)(*[int(f) for f in flags])

@@ -25,6 +25,8 @@ once; in that metric the transpose of L is covariant with P0^T = W P0 W^-1 (W = 
 off-diagonal entries), which is the projector for a transposed application in the standard
 representation. In the Wigner representation both directions use P0.
 """
+import warnings
+
 import numpy as np
 
 from . import _lib as L
@@ -101,12 +103,21 @@ class TranslationProjector:
             self.s_orb = torch.as_tensor(s_orb, device=self.dev)
             self.idx = torch.as_tensor(idx.ravel(), device=self.dev)
             self.nb = nb
-            C = torch.randn(3 * nat, 3 * nat, dtype=torch.float64, device=self.dev)
-            r = torch.randn(3 * nat, dtype=torch.float64, device=self.dev)
+            # self-check on a private generator (the caller's random stream is not touched)
+            gen = torch.Generator(device=self.dev).manual_seed(12345)
+            C = torch.randn(3 * nat, 3 * nat, dtype=torch.float64, device=self.dev,
+                            generator=gen)
+            r = torch.randn(3 * nat, dtype=torch.float64, device=self.dev, generator=gen)
             self.fold = True
             ok = (torch.allclose(self._avg_mat(C), self._avg_mat_gather(C), rtol=0, atol=1e-12)
                   and torch.allclose(self._avg_vec(r), r[self.perm].mean(0), rtol=0, atol=1e-12))
             self.fold = True if ok else None
+            if not ok:
+                warnings.warn("translation average: the orbit-folded form disagrees with the "
+                              "average over every translation; using the slower gather form")
+        else:
+            warnings.warn("translation average: the translations do not act freely on the "
+                          "atoms; using the gather form (O(n_t nat^2) per matrix)")
         n = self.n
         iu = np.triu_indices(n)
         self.iu = (torch.as_tensor(iu[0], device=self.dev), torch.as_tensor(iu[1], device=self.dev))
@@ -180,10 +191,15 @@ class GammaSectorOperator:
     (transposed standard-representation applications: psi in the sector of P0^T).
 
     sym: the symmetry blocks of all N_sym operations, in the order of (rotations, irt), the
-    supercell's space group; pols: (3 nat, n) polarization vectors of the kept modes."""
+    supercell's space group; pols: (3 nat, n) polarization vectors of the kept modes.
+
+    check_tol: every apply refuses (ValueError) a psi farther than this from the sector,
+    |psi - P0 psi| / |psi| (None: no check). A caller that has checked its vectors already
+    (run_FT checks the starting vector, and L keeps the sector) may skip the per-call check
+    inside ``with op.unchecked():``."""
 
     def __init__(self, X, Y, w, rho, T, sym: SymmetryBlocks, rotations, irt, pols,
-                 precisions=("fp64",), device=0, s_batch=0, check_tol=None):
+                 precisions=("fp64",), device=0, s_batch=0, check_tol=1e-8):
         cos = translation_cosets(rotations, irt)
         if cos is None:
             raise ValueError("the operations are not cosets of the pure translations (a space "
@@ -207,6 +223,20 @@ class GammaSectorOperator:
     @staticmethod
     def _adjoint(flags):
         return bool(flags & L.OP_TRANSPOSE) and not bool(flags & L.OP_WIGNER)
+
+    def unchecked(self):
+        """Context in which the applies skip the per-call sector check."""
+        import contextlib
+
+        @contextlib.contextmanager
+        def ctx():
+            saved = self.check_tol
+            self.check_tol = None
+            try:
+                yield self
+            finally:
+                self.check_tol = saved
+        return ctx()
 
     def _check(self, psi, flags):
         if self.check_tol is not None:

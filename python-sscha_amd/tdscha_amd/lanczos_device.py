@@ -14,16 +14,25 @@ from .ensemble import _stream_handle
 EPSILON = 1e-12   # the reference's stopping threshold (Py:121, Py:6144-6153)
 
 
-def truncate(a, b, c, s_norm=None):
+def truncate(a, b, c, s_norm=None, gs_norms=None):
     """Cut the coefficients where the reference's run_FT stops: a_k is kept, b_k / c_k of the
-    first step with |b_k| or |c_k| below 1e-12 are not (Py:6143-6153)."""
+    first step with |b_k| or |c_k| below 1e-12 are not (Py:6143-6153). gs_norms (full_gs runs):
+    min(|q|, |p|) of each step's new pair after the projection; below 1e-12 at step k the run
+    keeps b_k, c_k and ends after a_(k+1) (run_FT's next_converged, Py:6283-6303)."""
     a, b, c = (np.asarray(x, dtype=np.float64) for x in (a, b, c))
+    s = None if s_norm is None else np.asarray(s_norm)
     bad = np.nonzero((np.abs(b) < EPSILON) | (np.abs(c) < EPSILON) | ~np.isfinite(b) |
                      ~np.isfinite(c))[0]
-    if len(bad):
-        i = bad[0]
-        return a[:i + 1], b[:i], c[:i], (None if s_norm is None else np.asarray(s_norm)[:i])
-    return a, b, c, (None if s_norm is None else np.asarray(s_norm))
+    cut = len(b) if not len(bad) else int(bad[0])   # b, c kept below this index
+    n_a = cut + 1 if len(bad) else len(a)
+    if gs_norms is not None:
+        g = np.asarray(gs_norms, dtype=np.float64)[:cut]
+        dep = np.nonzero((g < EPSILON) | ~np.isfinite(g))[0]
+        if len(dep) and dep[0] + 1 < cut:
+            cut = int(dep[0]) + 1
+            n_a = cut + 1
+    n_a = min(n_a, len(a))
+    return a[:n_a], b[:cut], c[:cut], (None if s is None else s[:cut])
 
 
 class DeviceLanczos:
@@ -34,13 +43,14 @@ class DeviceLanczos:
     """
 
     def __init__(self, dev_ens, n_chains=1, precision="fp64", flags=None, shifts=None,
-                 symmetric=False, max_steps=100, keep_basis=False, mask=None):
+                 symmetric=False, max_steps=100, keep_basis=False, mask=None, full_gs=False):
         self.ens = dev_ens
         self.torch = dev_ens.torch
         self.nc = int(n_chains)
         self.max_steps = int(max_steps)
         self.symmetric = bool(symmetric)
-        self.keep_basis = bool(keep_basis)
+        self.full_gs = bool(full_gs)
+        self.keep_basis = bool(keep_basis) or self.full_gs
         n = dev_ens.n
         self.n_psi = n + n * (n + 1)
         flags = [0] * self.nc if flags is None else list(flags)
@@ -56,7 +66,7 @@ class DeviceLanczos:
         h = ctypes.c_void_p()
         rc = L.lib().tdscha_lanczos_create(ctypes.byref(h), dev_ens._h, L.PRECISIONS[precision],
                                            self.nc, fl, sh, int(symmetric), self.max_steps,
-                                           int(keep_basis), m)
+                                           int(self.keep_basis), int(self.full_gs), m)
         L.check(rc, "tdscha_lanczos_create")
         self._h = h
 
@@ -97,8 +107,16 @@ class DeviceLanczos:
         L.check(rc, "tdscha_lanczos_read")
         a, b, c, s = a[:k.value], b[:k.value], c[:k.value], s[:k.value]
         if truncated:
-            return truncate(a, b, c, s)
+            return truncate(a, b, c, s, self.gs_norms(chain) if self.full_gs else None)
         return a, b, c, s
+
+    def gs_norms(self, chain=0):
+        """min(|q|, |p|) of every step's new pair after the projection (full_gs runs)."""
+        k = ctypes.c_int()
+        g = np.zeros(self.max_steps)
+        L.check(L.lib().tdscha_lanczos_read_gs(self._h, int(chain), g.ctypes.data_as(L._dp),
+                                               ctypes.byref(k)), "tdscha_lanczos_read_gs")
+        return g[:k.value]
 
     def basis(self, chain, steps):
         """(Q, P) device views of the first steps + 1 stored vectors of a chain."""

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(lib, s), s
         assert s in L.SIGNATURES, s
-    assert lib.tdscha_abi_version() == 1
+    assert lib.tdscha_abi_version() == 2
 
 
 def _create(ens, **over):

@@ -129,3 +129,28 @@ def test_lanczos_gamma_sector_refuses_a_perturbation_outside_the_sector(ops):
     lz.psi = np.random.default_rng(8).standard_normal(len(lz.psi))
     with pytest.raises(ValueError, match="translation-invariant"):
         lz.run_FT(2, verbose=False)
+
+
+def test_public_applies_refuse_a_vector_outside_the_sector(ops):
+    """apply_full_L / apply_full_L_batched of the zone-centre operator (and the Lanczos
+    methods on it) raise on a psi outside the translation-invariant sector instead of
+    returning P0 L_P psi; inside ``unchecked()`` the caller takes that responsibility."""
+    ens, rots, irt, full, gam = ops
+    print("case: gamma operator applies on a random vector", flush=True)
+    n = len(ens.w)
+    psi = np.random.default_rng(21).standard_normal(n + n * (n + 1))
+    with pytest.raises(ValueError, match="translation-invariant"):
+        gam.apply_full_L(psi)
+    with pytest.raises(ValueError, match="translation-invariant"):
+        gam.apply_full_L_batched([psi])
+    lz = _lanczos(ens, rots, irt, True)
+    lz.psi = psi.copy()
+    for call in (lz.apply_full_L, lz.apply_L1_FT, lz.apply_anharmonic_FT):
+        with pytest.raises(ValueError, match="translation-invariant"):
+            lz.psi = psi.copy()
+            call()
+    with pytest.raises(ValueError, match="translation-invariant"):
+        lz.L_linop.matvec(psi)
+    with gam.unchecked():
+        gam.apply_full_L(psi)
+    assert gam.check_tol == 1e-8

@@ -1,8 +1,11 @@
 """Parity of the HIP path (through the C ABI) with the CPU oracle and the golden fixtures.
 
-Tolerance (FP64 path): 1e-10, norm-wise, relative to the oracle's absolute-value sums
-(the sum of |every term|), which is the scale that bounds the rounding error of any
-summation order; for outputs that do not cancel this equals a plain relative error.
+Tolerance (FP64 path): plain relative error ||out - ref|| / ||ref|| <= 1e-10 for every output
+that does not cancel (||ref|| at least 1e-6 of the oracle's absolute-value sums, the sum of
+|every term|). An output that cancels to rounding (f_av of a perturbation that the symmetry
+makes vanish) must itself vanish: ||out|| <= 1e-12 of those sums. Reduced-precision forms are
+measured against the absolute-value sums, the scale their rounding error follows.
+Full L: plain relative 1e-10 on the whole vector and on its anharmonic part alone.
 """
 import numpy as np
 import pytest
@@ -27,6 +30,23 @@ def _scaled_err(out, ref, scale):
     return np.linalg.norm(out - ref) / max(np.linalg.norm(scale), 1e-300)
 
 
+CANCEL = 1e-6     # ||ref|| below this fraction of the abs-sum scale: the output cancels
+VANISH = 1e-12    # a cancelling output must vanish to this fraction of the scale
+
+
+def _assert_fp64(out, ref, scale, what):
+    """Plain relative error for a non-cancelling output, vanishing for a cancelling one."""
+    nref, nsc = np.linalg.norm(ref), np.linalg.norm(scale)
+    if nref >= CANCEL * nsc:
+        err = np.linalg.norm(out - ref) / nref
+        print("%s: relative error %.3e (||ref|| / scale %.1e)" % (what, err, nref / nsc))
+        assert err <= TOL64, (what, err)
+    else:
+        v = np.linalg.norm(out) / nsc
+        print("%s: cancels (||ref|| / scale %.1e), ||out|| / scale %.1e" % (what, nref / nsc, v))
+        assert v <= VANISH, (what, v)
+
+
 def _check(ens, R1, Y1, d4, dev=None, tol=TOL64, precision="fp64"):
     f_o, d_o, fa, da = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, d4,
                                                ens.sym, with_abs=True)
@@ -35,8 +55,12 @@ def _check(ens, R1, Y1, d4, dev=None, tol=TOL64, precision="fp64"):
     f, d = dev.apply(R1, Y1, terms=terms, precision=precision)
     f, d = f.cpu().numpy(), d.cpu().numpy()
     ef, ed = _scaled_err(f, f_o, fa), _scaled_err(d, d_o, da)
-    print("f err %.3e  d2v err %.3e" % (ef, ed))
-    assert ef < tol and ed < tol, (ef, ed)
+    print("f err %.3e  d2v err %.3e (of the abs-sum scale)" % (ef, ed))
+    if precision == "fp64" and tol == TOL64:
+        _assert_fp64(f, f_o, fa, "f_av")
+        _assert_fp64(d, d_o, da, "d2v")
+    else:
+        assert ef < tol and ed < tol, (ef, ed)
     return f, d
 
 
@@ -49,10 +73,9 @@ def test_mode10_golden_reference_vectors(mode10):
         _, _, fa, da = orc.perturb_average_sym(mode10.X, mode10.Y, mode10.w, mode10.rho, R1, Y1,
                                                mode10.T, d4, mode10.sym, with_abs=True)
         f, d = dev.apply(R1, Y1, terms=1 | 2 | (4 if d4 else 0))
-        ef = _scaled_err(f.cpu().numpy(), kv["f_%d" % k], fa)
-        ed = _scaled_err(d.cpu().numpy(), kv["d2v_%d" % k], da)
-        print("case", k, ef, ed)
-        assert ef < TOL64 and ed < TOL64
+        print("case", k)
+        _assert_fp64(f.cpu().numpy(), kv["f_%d" % k], fa, "f_av")
+        _assert_fp64(d.cpu().numpy(), kv["d2v_%d" % k], da, "d2v")
 
 
 @pytest.mark.parametrize("seed,n,N,n_sym,d4", [
@@ -168,8 +191,8 @@ def test_trio_entry_points(mode10):
     lib.get_f_average_from_Y_pert_sym_fast(*common(Y1c), f.ctypes.data_as(L._dp))
     lib.get_d2v_dR2_from_R_pert_sym_fast(*common(R1c), d.ctypes.data_as(L._dp))
     lib.get_d2v_dR2_from_Y_pert_sym_fast(*common(Y1c), d.ctypes.data_as(L._dp))
-    assert _scaled_err(f, f_o, fa) < TOL64
-    assert _scaled_err(d, d_o, da) < TOL64
+    _assert_fp64(f, f_o, fa, "f_av")
+    _assert_fp64(d, d_o, da, "d2v")
 
 
 def test_get_perturb_average_sym_shim(mode10):
@@ -182,8 +205,8 @@ def test_get_perturb_average_sym_shim(mode10):
     d = np.zeros((45, 45))
     GetPerturbAverageSym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 0, 384, ens.sym.symmetries,
                          ens.sym.N_degeneracy, ens.sym.degenerate_space, ens.sym.block_id, f, d)
-    assert _scaled_err(f, f_o, fa) < TOL64
-    assert _scaled_err(d, d_o, da) < TOL64
+    _assert_fp64(f, f_o, fa, "f_av")
+    _assert_fp64(d, d_o, da, "d2v")
     with pytest.raises(ValueError):
         GetPerturbAverageSym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 2, 384,
                              ens.sym.symmetries, ens.sym.N_degeneracy, ens.sym.degenerate_space,
@@ -230,6 +253,26 @@ def test_full_L_recurrence_standard(mode10):
         err_o = np.linalg.norm(Lq - Lq_o) / np.linalg.norm(Lq_o)
         print(k, err, err_o)
         assert err < 1e-10 and err_o < TOL64
+
+
+def test_anharmonic_part_alone_matches_oracle(mode10):
+    """The anharmonic part of L (L - L1) on its own at 1e-10 relative: on q_0 it is only
+    ~1e-4 of ||L q_0||, so the whole-vector check above does not bound it. Standard and
+    transposed on the stored bases, Wigner on its own basis."""
+    from tdscha_amd import _lib as L
+    st = fixtures.load_npz("lanczos_std.npz")
+    sw = fixtures.load_npz("lanczos_wigner.npz")
+    dev = _dev_ens(mode10)
+    cases = [(st["Q"][k], 0, dict()) for k in range(10)]
+    cases += [(st["P"][k], L.OP_TRANSPOSE, dict(transpose=True)) for k in (0, 4, 9)]
+    cases += [(sw["Q"][k], L.OP_WIGNER, dict(use_wigner=True)) for k in (0, 5, 9)]
+    for v, fl, kw in cases:
+        out = dev.apply_full_L(v, flags=fl | L.OP_IGNORE_HARMONIC).cpu().numpy()
+        ref = orc.apply_anharmonic_FT(mode10, v, **kw)
+        err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        ratio = np.linalg.norm(ref) / np.linalg.norm(orc.apply_full_L(mode10, v, **kw))
+        print(kw, "anharmonic / full %.1e, relative error %.2e" % (ratio, err))
+        assert err < TOL64, (kw, err)
 
 
 def test_full_L_transpose_matches_oracle(mode10):
