@@ -28,6 +28,8 @@
 #ifndef TDSCHA_HIP_H
 #define TDSCHA_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -114,6 +116,28 @@ int tdscha_get_perturb_average_sym(const double *X, const double *Y, const doubl
                                    const int *const *sym_basis_list, const int *blocks_ids,
                                    int N_configs, int N_modes, double *force_out,
                                    double *d2v_out);
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * arguments, contract and FP64 result as tdscha_get_perturb_average_sym, computed with no
+ * hipMalloc / hipFree and no allocating library call. Every device buffer of the call is
+ * carved from a static device scratch array of tdscha_noalloc_scratch_bytes() (256 MiB),
+ * held by each GPU the library runs on until the process ends; the symmetry images are
+ * processed in fewer slots per pass when the workspace would not fit (results do not
+ * depend on it). An ensemble that does not fit even with one slot per pass is rejected
+ * with code 5 before any upload. All calls on one GPU share that GPU's scratch: calls that
+ * overlap on one GPU (from several host threads) race and must be serialised by the
+ * caller. Runs on the current device, synchronously. */
+int tdscha_get_perturb_average_sym_noalloc(const double *X, const double *Y, const double *w,
+                                           const double *rho, const double *R1,
+                                           const double *Y1, double T, int apply_D4,
+                                           int N_syms, int n_blocks,
+                                           const double *const *sym_list, const int *n_deg,
+                                           const int *const *sym_basis_list,
+                                           const int *blocks_ids, int N_configs, int N_modes,
+                                           double *force_out, double *d2v_out);
+/* Size in bytes of the per-GPU scratch array of the entry point above. */
+size_t tdscha_noalloc_scratch_bytes(void);
 
 /* ---- Device-resident ensemble (the hot path of a Lanczos run) ------------------------
  * No counterpart in the reference, which re-reads X, Y and the symmetries on every call

@@ -24,6 +24,11 @@ struct tdscha_ensemble {
   Ensemble e;
   std::vector<void *> allocs;
   long long bytes = 0;
+  // When arena is set, every device buffer is carved from that region (the no-allocation
+  // entry point's static scratch) instead of hipMalloc, and nothing is freed.
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool arena_overflow = false;
 };
 
 #define TD_CHECK(expr)                                                                    \
@@ -43,6 +48,19 @@ int dev_alloc(tdscha_ensemble *h, T **p, size_t count) {
   void *ptr = nullptr;
   size_t bytes = count * sizeof(T);
   if (bytes == 0) bytes = sizeof(T);
+  if (h->arena) {
+    const size_t off = (h->arena_used + 255) & ~(size_t)255;
+    if (off > h->arena_cap || bytes > h->arena_cap - off) {
+      h->arena_overflow = true;
+      set_error("the buffers do not fit the " + std::to_string(h->arena_cap >> 20) +
+                " MiB scratch region");
+      return 12;
+    }
+    h->arena_used = off + bytes;
+    h->bytes += (long long)bytes;
+    *p = reinterpret_cast<T *>(h->arena + off);
+    return 0;
+  }
   hipError_t err = hipMalloc(&ptr, bytes);
   if (err != hipSuccess) {
     set_error(std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
@@ -358,11 +376,19 @@ int tdscha_profile_query(int group, double *total_ms, long long *count) {
   return 0;
 }
 
-int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, const double *Y,
-                           const double *w, const double *rho, double T, int n_modes, int n_configs,
-                           int N_sym, int n_blocks, const double *const *symmetries,
-                           const int *N_degeneracy, const int *const *degenerate_space,
-                           const int *blocks_ids, int precisions, int s_batch) {
+}  // extern "C"
+
+namespace {
+
+// Upload an ensemble. arena != nullptr: every device buffer comes from [arena, arena + cap)
+// (no hipMalloc); *sb_used receives the slot count the workspace was sized for; a region
+// too small for that workspace returns 12, one too small for the operands alone 13.
+int ensemble_create_impl(tdscha_ensemble **out, int device, const double *X, const double *Y,
+                         const double *w, const double *rho, double T, int n_modes,
+                         int n_configs, int N_sym, int n_blocks,
+                         const double *const *symmetries, const int *N_degeneracy,
+                         const int *const *degenerate_space, const int *blocks_ids,
+                         int precisions, int s_batch, char *arena, size_t cap, int *sb_used) {
   if (!out) { set_error("out is NULL"); return 1; }
   *out = nullptr;
   if (n_modes <= 0 || n_configs <= 0 || N_sym <= 0) {
@@ -380,6 +406,8 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
     }
   tdscha_ensemble *h = new (std::nothrow) tdscha_ensemble();
   if (!h) { set_error("out of host memory"); return 11; }
+  h->arena = arena;
+  h->arena_cap = cap;
   Ensemble &e = h->e;
   int rc = check_blocks(n_modes, n_blocks, N_degeneracy, degenerate_space, blocks_ids, w, e);
   if (rc) { delete h; return rc; }
@@ -397,6 +425,7 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   for (int i = 0; i < n_configs; ++i) neff += rho[i];
   e.N_eff = neff;
   e.s_batch = choose_s_batch(e, s_batch);
+  if (sb_used) *sb_used = e.s_batch;
 
   const int N = e.N, n = e.n, Np = e.Np, npd = e.npd;
   std::vector<double> fu(npd, 0.0), fp(npd, 0.0), nocc(n, 0.0);
@@ -453,6 +482,7 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   e.n_pairs = (int)hpairs.size();
   rc |= dev_alloc(h, &e.dgroups, hgroups.size());
   rc |= dev_alloc(h, &e.dpairs, hpairs.size());
+  const bool fixed_overflow = h->arena_overflow;   // before any per-slot buffer
   if (!rc) rc |= alloc_slots(h, e.s_batch);
   if (!rc) rc |= ensure_chains(h, 1);
   const bool want32 = e.prec_mask & (1 << kFP32), wantbf = e.prec_mask & (1 << kBF16x3);
@@ -471,7 +501,11 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
     e.X_lo = e.X_hi + NX;
     e.XtT_lo = e.XtT_hi + NX;
   }
-  if (rc) { tdscha_ensemble_destroy(h); return 11; }
+  if (rc) {
+    const bool overflow = h->arena_overflow;
+    tdscha_ensemble_destroy(h);
+    return overflow ? (fixed_overflow ? 13 : 12) : 11;
+  }
 
   auto up = [&](void *dst, const void *src, size_t bytes) -> int {
     hipError_t err = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
@@ -527,10 +561,25 @@ int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, c
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int tdscha_ensemble_create(tdscha_ensemble **out, int device, const double *X, const double *Y,
+                           const double *w, const double *rho, double T, int n_modes, int n_configs,
+                           int N_sym, int n_blocks, const double *const *symmetries,
+                           const int *N_degeneracy, const int *const *degenerate_space,
+                           const int *blocks_ids, int precisions, int s_batch) {
+  return ensemble_create_impl(out, device, X, Y, w, rho, T, n_modes, n_configs, N_sym, n_blocks,
+                              symmetries, N_degeneracy, degenerate_space, blocks_ids, precisions,
+                              s_batch, nullptr, 0, nullptr);
+}
+
 int tdscha_ensemble_destroy(tdscha_ensemble *h) {
   if (!h) return 0;
   (void)hipSetDevice(h->e.device);
-  for (void *p : h->allocs) (void)hipFree(p);
+  if (!h->arena)
+    for (void *p : h->allocs) (void)hipFree(p);
   delete h;
   return 0;
 }
@@ -885,6 +934,136 @@ int tdscha_get_perturb_average_sym(const double *X, const double *Y, const doubl
   int terms = kTermF | kTermD2vR | (apply_D4 ? kTermD2vY : 0);
   return host_call(X, Y, w, rho, R1, Y1, T, terms, N_modes, N_configs, N_syms, n_blocks, sym_list,
                    n_deg, sym_basis_list, blocks_ids, force_out, d2v_out, 0);
+}
+
+// ---- the no-allocation form of tdscha_get_perturb_average_sym ---------------------------
+// Every device buffer of the call (the ensemble's operands, the per-pass workspace, the
+// perturbation and the outputs) is carved from `scratch`, a file-scope device array of
+// kScratchBytes that each GPU holds for the life of the process. In host code the array's
+// name is only a stand-in for the symbol: its device address comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is used.
+
+}  // extern "C"
+
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array (looked up on the first call on a device).
+int scratch_for_device(int dev, char **out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index " + std::to_string(dev) + " out of range");
+    return 1;
+  }
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(err));
+      return 10;
+    }
+    g_scratch_ptr[dev] = static_cast<char *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t tdscha_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int tdscha_get_perturb_average_sym_noalloc(const double *X, const double *Y, const double *w,
+                                           const double *rho, const double *R1, const double *Y1,
+                                           double T, int apply_D4, int N_syms, int n_blocks,
+                                           const double *const *sym_list, const int *n_deg,
+                                           const int *const *sym_basis_list,
+                                           const int *blocks_ids, int N_configs, int N_modes,
+                                           double *force_out, double *d2v_out) {
+  if (apply_D4 != 0 && apply_D4 != 1) {
+    set_error("the apply_D4 value passed must be either 0 or 1");
+    return 2;
+  }
+  if (!X || !Y || !w || !rho || !force_out || !d2v_out) {
+    set_error("NULL argument");
+    return 1;
+  }
+  if (N_modes <= 0 || N_configs <= 0 || N_syms <= 0) {
+    set_error("n_modes, n_configs and N_sym must be positive");
+    return 1;
+  }
+  const int terms = kTermF | kTermD2vR | (apply_D4 ? kTermD2vY : 0);
+  int dev = 0;
+  TD_CHECK(hipGetDevice(&dev));
+  char *base = nullptr;
+  int rc = scratch_for_device(dev, &base);
+  if (rc) return rc;
+  // per-call buffers first: R1 (n), Y1 (n x n), f (n), d2v (n x n), each 256-byte aligned
+  const size_t n = (size_t)N_modes, nn = n * n;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t off_y1 = al(8 * n), off_f = off_y1 + al(8 * nn), off_d = off_f + al(8 * n);
+  const size_t call_bytes = off_d + al(8 * nn);
+  if (call_bytes >= kScratchBytes) {
+    set_error("n_modes = " + std::to_string(N_modes) + " exceeds what the " +
+              std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch holds");
+    return 5;
+  }
+  double *dR1 = reinterpret_cast<double *>(base);
+  double *dY1 = reinterpret_cast<double *>(base + off_y1);
+  double *df = reinterpret_cast<double *>(base + off_f);
+  double *dd = reinterpret_cast<double *>(base + off_d);
+  // the ensemble in the rest; fewer symmetry slots per pass when the workspace does not fit
+  // (the result does not depend on the pass size)
+  const bool prof = profile_swap(false);   // no timing events: they would be created on demand
+  tdscha_ensemble *h = nullptr;
+  int sb = 0, sb_used = 0;
+  for (;;) {
+    rc = ensemble_create_impl(&h, dev, X, Y, w, rho, T, N_modes, N_configs, N_syms, n_blocks,
+                              sym_list, n_deg, sym_basis_list, blocks_ids, 1, sb,
+                              base + call_bytes, kScratchBytes - call_bytes, &sb_used);
+    if (rc != 12 || sb_used <= 1) break;
+    sb = sb_used / 2;
+  }
+  if (rc == 12 || rc == 13) {
+    set_error("the ensemble (n_modes " + std::to_string(N_modes) + ", n_configs " +
+              std::to_string(N_configs) + ", N_sym " + std::to_string(N_syms) +
+              ") exceeds what the " + std::to_string(kScratchBytes >> 20) +
+              " MiB no-allocation scratch holds");
+    profile_swap(prof);
+    return 5;
+  }
+  if (rc) { profile_swap(prof); return rc; }
+  hipError_t err = R1 ? hipMemcpy(dR1, R1, n * 8, hipMemcpyHostToDevice)
+                      : hipMemset(dR1, 0, n * 8);
+  if (err == hipSuccess)
+    err = Y1 ? hipMemcpy(dY1, Y1, nn * 8, hipMemcpyHostToDevice) : hipMemset(dY1, 0, nn * 8);
+  if (err != hipSuccess) {
+    set_error(std::string("input copy failed: ") + hipGetErrorString(err));
+    rc = 10;
+  }
+  if (!rc) rc = tdscha_ensemble_apply(h, kFP64, terms, dR1, dY1, df, dd, 0, nullptr);
+  if (!rc) {
+    err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(force_out, df, n * 8, hipMemcpyDeviceToHost);
+    if (err == hipSuccess) err = hipMemcpy(d2v_out, dd, nn * 8, hipMemcpyDeviceToHost);
+    if (err != hipSuccess) {
+      set_error(std::string("device execution failed: ") + hipGetErrorString(err));
+      rc = 10;
+    }
+  }
+  tdscha_ensemble_destroy(h);   // host bookkeeping only: arena buffers are not freed
+  profile_swap(prof);
+  return rc;
 }
 
 static void die_on(int rc, const char *fn) {

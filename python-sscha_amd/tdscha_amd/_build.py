@@ -39,7 +39,7 @@ def build(verbose=False):
     def compile_one(src):
         path = os.path.join(CSRC, src)
         obj = os.path.join(obj_dir, src + ".o")
-        lang = ["-x", "hip"] if src.endswith(".hip") else []
+        lang = ["-x", "hip"]   # tdscha_hip.cpp holds a device array (the no-allocation scratch)
         cmd = [cc] + lang + common + DEVICE_FLAGS + ["-c", path, "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -35,6 +35,10 @@ SIGNATURES = {
     "tdscha_get_perturb_average_sym": (ctypes.c_int, [
         _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "tdscha_get_perturb_average_sym_noalloc": (ctypes.c_int, [
+        _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "tdscha_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "tdscha_ensemble_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int]),
