@@ -16,9 +16,12 @@ each kernel group over the timed steps, for the contraction with the larger shar
 time; `roofline_other_contraction` gives the other one. `frac` counts algorithmic flops
 (FLOP_L = 6 N N_sym n^2, GEMM1 2/6 of it, GEMM2 4/6); `frac_executed` the flops the kernel
 executes (GEMM2 forms B_s once and uses B_s + B_s^T, half its algorithmic count).
-`cpu_baseline` times the reference's own C trio
-(oracle/_ref, built from CModules/LanczosFunctions.c) on one host core over a bounded
-sample of images of the same ensemble and scales it linearly to the full image count.
+`fp64_plain` is the plain FP64 path on the same ensemble: one L application on one vector
+over every image, timed once, with its contractions' event times. `cpu_baseline` times the
+reference's own C trio (oracle/_ref, built from CModules/LanczosFunctions.c) on the host
+cores as independent processes on configuration slices (the reference's MODE_FAST_MPI
+split) over a bounded sample of images of the same ensemble, scaled linearly to the full
+image count.
 
 `gamma_sector` is a separate line, not the headline: the same two chains projected on the
 translation-invariant sector (the zone-centre perturbations of IR and Raman spectra live
@@ -78,8 +81,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ens, seconds=12.0):
-    """Reference C trio on one core over 1 config x k symmetries, scaled to all images."""
+def _ref_slice_worker(X, Y, w, rho, R1, Y1, T, blocks, barrier, queue):
+    """One MODE_FAST_MPI rank: the reference C trio on its configuration slice (a child
+    process started with spawn; it never touches the GPU)."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    from oracle import oracle as orc
+    sym = orc.SymBlocks(*blocks)
+    orc.ref_perturb_average_sym(X[:1], Y[:1], w, rho[:1], R1, Y1, T, 1, sym.subset(1))
+    barrier.wait()
+    t0 = time.perf_counter()
+    orc.ref_perturb_average_sym(X, Y, w, rho, R1, Y1, T, 1, sym)
+    queue.put(time.perf_counter() - t0)
+
+
+def cpu_baseline(ens, per_process_seconds=2.0, max_procs=16):
+    """The reference C trio (oracle/_ref, compiled from CModules/LanczosFunctions.c) on the
+    host cores, MODE_FAST_MPI style: P independent processes, each on its own configuration
+    with the same first k symmetries (the reference's MPI split is over configurations,
+    LanczosFunctions.c:2123-2136), all started together; the rate is P k images over the
+    slowest process's time, scaled linearly to N N_sym images (the trio's cost is linear in
+    the image count)."""
+    import multiprocessing as mp
     import numpy as np
     from oracle import oracle as orc
     if not orc.ref_available():
@@ -90,25 +112,50 @@ def cpu_baseline(ens, seconds=12.0):
     A = rng.standard_normal((n, n)) * 1e-4
     Y1 = A + A.T
     sym = orc.SymBlocks(ens.symmetries, ens.N_degeneracy, ens.degenerate_space, ens.block_id)
+    os.environ["OMP_NUM_THREADS"] = "1"
+    # one core: per-image time on 1 configuration
     k = 1
-    t = 0.0
     while True:
-        os.environ["OMP_NUM_THREADS"] = "1"
         t0 = time.perf_counter()
         orc.ref_perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, 1, sym,
                                     n_configs=1, n_syms=k)
-        t = time.perf_counter() - t0
-        if t > seconds / 4 or k >= ens.n_syms:
+        t1 = time.perf_counter() - t0
+        if t1 > 0.5 or k >= ens.n_syms:
             break
-        k = min(ens.n_syms, max(k * 2, int(k * (seconds / 4) / max(t, 1e-3))))
-    per_image = t / k
+        k = min(ens.n_syms, max(2 * k, int(k * 0.6 / max(t1, 1e-3))))
+    per_image_1 = t1 / k
+    k = max(1, min(ens.n_syms, int(per_process_seconds / per_image_1)))
+    try:
+        P = len(os.sched_getaffinity(0))
+    except AttributeError:
+        P = os.cpu_count() or 1
+    P = max(1, min(P, max_procs, ens.X.shape[0]))
+    sub = sym.subset(k)
+    blocks = (sub.symmetries, sub.N_degeneracy, sub.degenerate_space, sub.block_id)
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(P)
+    queue = ctx.Queue()
+    procs = [ctx.Process(target=_ref_slice_worker,
+                         args=(ens.X[r:r + 1].copy(), ens.Y[r:r + 1].copy(), ens.w,
+                               ens.rho[r:r + 1].copy(), R1, Y1, ens.T, blocks, barrier, queue))
+             for r in range(P)]
+    for pr in procs:
+        pr.start()
+    times = [queue.get(timeout=600) for _ in procs]
+    for pr in procs:
+        pr.join()
+    wall = max(times)
+    rate = P * k / wall                      # images per second on P cores
     images = ens.X.shape[0] * ens.n_syms
-    return {"value": 1.0 / (per_image * images), "unit": "L-applications/s", "cores": 1,
+    return {"value": rate / images, "unit": "L-applications/s", "cores": P,
             "kind": "reference",
-            "sample": "reference C trio (get_f_average / get_d2v_dR2 from R / from Y, "
-                      "-O2 serial) on 1 config x %d symmetries (%.1f s, %.3g s per image), "
-                      "scaled linearly to %d configs x %d symmetries" % (
-                          k, t, per_image, ens.X.shape[0], ens.n_syms)}
+            "sample": "reference C trio (get_f_average / get_d2v_dR2 from R / from Y, -O2, "
+                      "serial per process) as %d independent processes (the MODE_FAST_MPI "
+                      "split over configurations), each 1 config x the first %d symmetries "
+                      "(slowest %.2f s, %.1f s of CPU work in all; one core alone: %.3g s per "
+                      "image), scaled linearly to %d configs x %d symmetries" % (
+                          P, k, wall, sum(times), per_image_1, ens.X.shape[0], ens.n_syms),
+            "single_core_s_per_image": per_image_1}
 
 
 def gamma_sector_line(args, ens, sym, precs, dens, psis, flags, shifts, full_value, reps=5):
@@ -179,8 +226,8 @@ def main():
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB on the device "
                          "(a kernel of known traffic for hardware-counter calibration)")
-    ap.add_argument("--fp64-sample-syms", type=int, default=-1,
-                    help="symmetries of the FP64 comparison sample (-1: N_sym/32, 0: skip)")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the plain FP64 comparison application")
     ap.add_argument("--no-gamma", action="store_true",
                     help="skip the zone-centre (translation-invariant) operator line")
     args = ap.parse_args()
@@ -297,30 +344,41 @@ def main():
             log("zone-centre operator line failed: %s" % e)
             gamma = {"error": str(e)}
     fp64_ref = None
-    if args.precision != "fp64" and args.fp64_sample_syms != 0:
-        # the plain FP64 path on the first k symmetries: its cost is linear in the number of
-        # symmetry passes, so the full-ensemble rate is the sample rate times k / N_sym
-        k = args.fp64_sample_syms if args.fp64_sample_syms > 0 else max(1, ens.n_syms // 32)
-        k = min(k, ens.n_syms)
-        sub = SymmetryBlocks([s[:k] for s in ens.symmetries], ens.N_degeneracy,
-                             ens.degenerate_space, ens.block_id)
+    if args.precision != "fp64" and not args.no_fp64:
+        # the plain FP64 path: one L application on one vector over every image, timed once
+        # (about 30 s at the default workload), with the contractions timed by events
         del dens
         torch.cuda.empty_cache()
-        d64 = DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sub, s_batch=args.s_batch)
-        d64.apply_full_L(psis[0], precision="fp64", out=outs[0])
+        d64 = DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T, sym, s_batch=args.s_batch)
+        log("fp64 plain: device ensemble %.2f GB, s_batch %d" % (d64.nbytes / 1e9,
+                                                                  d64.s_batch))
+        L.lib().tdscha_profile_reset()
+        L.lib().tdscha_profile_enable(1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         d64.apply_full_L(psis[0], precision="fp64", out=outs[0])
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
-        v64 = 1.0 / (t64 * ens.n_syms / k)
+        L.lib().tdscha_profile_enable(0)
+        g64 = {name: L.profile_query(gid) for gid, name in L.KGROUP_NAMES.items()}
+        v64 = 1.0 / t64
+        rl64 = {}
+        for grp in ("gemm1", "gemm2"):
+            ms, cnt = g64[grp]
+            if cnt:
+                ach = flop_l * share[grp] / cnt / (ms / cnt * 1e-3) / 1e12
+                rl64[grp] = {"kernel": "k_%s<double>" % grp, "ms_per_launch": round(ms / cnt, 3),
+                             "launches": cnt, "achieved": round(ach, 3),
+                             "frac": round(ach / PEAK_TFLOPS["fp64"], 4),
+                             "frac_executed": round(ach * executed[grp] /
+                                                    PEAK_TFLOPS["fp64"], 4)}
         fp64_ref = {"value": v64, "unit": "L-applications/s",
-                    "sample": "plain FP64 path, one L application over the first %d of %d "
-                              "symmetries (%.3f s), scaled by %d/%d" % (k, ens.n_syms, t64,
-                                                                      ens.n_syms, k),
-                    "speedup": value / v64}
-        log("fp64 plain: %.4g L/s (sample %d syms, %.3f s) -> speedup %.2fx" % (
-            v64, k, t64, value / v64))
+                    "sample": "plain FP64 path, one L application on one vector over all %d "
+                              "configs x %d symmetries (%.2f s, timed once)" % (
+                                  N, ens.n_syms, t64),
+                    "speedup": value / v64, "kernel_groups_ms": g64, "roofline": rl64}
+        log("fp64 plain: %.4g L/s (%.2f s) -> speedup %.2fx; groups %s" % (
+            v64, t64, value / v64, g64))
         d64.close()
     base = None
     if not args.no_cpu_baseline:
