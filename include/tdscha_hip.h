@@ -216,10 +216,15 @@ int tdscha_ensemble_reserve(tdscha_ensemble *ens, int n_chains);
  * where A_p is the anharmonic part of the operator of tdscha_apply_full_L with flags[p]
  * (TRANSPOSE / WIGNER / REVERSE / IGNORE_V3 / IGNORE_V4 as there; IGNORE_V4 equal for all
  * pairs; the harmonic part and the shift do not depend on Y). grad: device, N x n
- * row-major FP64 (the layout of Y); written (accumulate = 0) or added to. FP64 throughout. */
-int tdscha_ensemble_force_grad(tdscha_ensemble *ens, int n_pairs, const int *flags,
-                               const double *const *u, const double *const *v, double *grad,
-                               int accumulate, void *stream);
+ * row-major FP64 (the layout of Y); written (accumulate = 0) or added to.
+ * precision TDSCHA_PREC_FP64: FP64 MFMA throughout. TDSCHA_PREC_FP32 (ensemble created with
+ * the FP32 copies): the three contractions per pass (the forward weights, the seeds' row
+ * sums and the adjoint product itself) in FP32 MFMA with FP32 accumulation inside a pass;
+ * rotations, weights and the sum over passes in FP64. */
+int tdscha_ensemble_force_grad(tdscha_ensemble *ens, int precision, int n_pairs,
+                               const int *flags, const double *const *u,
+                               const double *const *v, double *grad, int accumulate,
+                               void *stream);
 
 /* ---- Device-resident Lanczos (the recurrence of Lanczos.run_FT, Py:5843-6372) ---------
  * n_chains starting vectors advance together: every step is one batched L application

@@ -406,9 +406,13 @@ __global__ void k_fav_reduce(const double *__restrict__ Vp, double *__restrict__
 // GEMM1. One K-concatenated FP64 MFMA product per
 // (configs x modes) tile: for every slot of the pass the two row-scaled products
 // (2 c1 X)(U~ f_ups) and (k X)(Ys f_psi), then the 2 sb rank-1 terms as extra k columns.
+// Force adjoint, in the arithmetic type T of the contraction (FP64, or FP32 with FP32
+// accumulation inside a launch; the launches' sums are added in FP64): X, Us and Ys in T,
+// every weight and seed in FP64, rounded to T as it is staged.
+template <class T>
 struct AdjArgs {
-  const double *X;              // [Np x npd]
-  const double *Us, *Ys;        // [sb x npd x npd]  f_ups S^T U S f_ups, S^T Ysym S
+  const T *X;                   // [Np x npd]
+  const T *Us, *Ys;             // [sb x npd x npd]  f_ups S^T U S f_ups, S^T Ysym S
   const double *W, *Wu;         // [sb x 4 x Np]     forward weights, (mu, -, nu, -)
   const double *G, *Gu;         // [sb x 2 x npd]    (f_ups S^T R1, S^T R1 / f_psi), (., S^T uR)
   const double *rho, *fups, *fpsi;
@@ -417,8 +421,9 @@ struct AdjArgs {
   double kf, kr, ky;
 };
 
-__device__ inline void adj_load(const AdjArgs &g, int step, int i0, int c0, double (&ra)[4],
-                                double (&rb)[4]) {
+template <class T>
+__device__ inline void adj_load(const AdjArgs<T> &g, int step, int i0, int c0, T (&ra)[4],
+                                T (&rb)[4]) {
   const int t = threadIdx.x, row = t >> 2, kk = (t & 3) * 4;
   const int i = i0 + row, c = c0 + row;
   const int KT = g.npd / BK;
@@ -429,56 +434,58 @@ __device__ inline void adj_load(const AdjArgs &g, int step, int i0, int c0, doub
     const double ca = half == 0 ? 2.0 * Ws[i]
                                 : g.rho[i] * (-g.kf * Wus[2 * (size_t)g.Np + i] / 3 +
                                               g.ky * Wus[i] / 4);
-    const double *xr = g.X + (size_t)i * g.npd + k0;
-    const double *mr = (half == 0 ? g.Us : g.Ys) + (size_t)slot * g.npd * g.npd +
-                       (size_t)c * g.npd + k0;
+    const T *xr = g.X + (size_t)i * g.npd + k0;
+    const T *mr = (half == 0 ? g.Us : g.Ys) + (size_t)slot * g.npd * g.npd +
+                  (size_t)c * g.npd + k0;
     const double cb = half == 0 ? (c < g.n ? 1.0 / g.fups[c] : 0.0) : g.fpsi[c];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      ra[j] = ca * xr[j];
-      rb[j] = cb * mr[j];
+      ra[j] = (T)(ca * (double)xr[j]);
+      rb[j] = (T)(cb * (double)mr[j]);
     }
   } else {
     const int k0 = (step - main_steps) * BK + kk;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kap = k0 + j, slot = kap >> 1;
-      ra[j] = rb[j] = 0.0;
+      ra[j] = rb[j] = (T)0;
       if (slot < g.sb) {
         const double *Ws = g.W + (size_t)slot * 4 * g.Np, *Wus = g.Wu + (size_t)slot * 4 * g.Np;
         if ((kap & 1) == 0) {
-          ra[j] = Ws[2 * (size_t)g.Np + i];                        // e1
-          rb[j] = g.Gu[(size_t)slot * 2 * g.npd + g.npd + c];      // S^T uR
+          ra[j] = (T)Ws[2 * (size_t)g.Np + i];                        // e1
+          rb[j] = (T)g.Gu[(size_t)slot * 2 * g.npd + g.npd + c];      // S^T uR
         } else {
-          ra[j] = -g.kr * g.rho[i] * Wus[i] / 3;                   // -kr rho mu / 3
-          rb[j] = g.G[(size_t)slot * 2 * g.npd + g.npd + c] * g.fpsi[c];   // S^T R1
+          ra[j] = (T)(-g.kr * g.rho[i] * Wus[i] / 3);                 // -kr rho mu / 3
+          rb[j] = (T)(g.G[(size_t)slot * 2 * g.npd + g.npd + c] * g.fpsi[c]);   // S^T R1
         }
       }
     }
   }
 }
 
-__global__ __launch_bounds__(256, 2) void k_adj_grad(AdjArgs g) {
-  __shared__ double As[2][BK * LDS_LD];
-  __shared__ double Bs[2][BK * LDS_LD];
+template <class T>
+__global__ __launch_bounds__(256, 2) void k_adj_grad(AdjArgs<T> g) {
+  typedef typename Acc<T>::type V;
+  __shared__ T As[2][BK * LDS_LD];
+  __shared__ T Bs[2][BK * LDS_LD];
   const int i0 = blockIdx.x * BM, c0 = blockIdx.y * BN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = w >> 1, wc = w & 1;
   const int lr = l & 15, lg = l >> 4;
-  v4d acc[2][2];
+  V acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; ++b) acc[a][b] = (V){0, 0, 0, 0};
   const int steps = g.sb * 2 * (g.npd / BK) + (2 * g.sb + BK - 1) / BK;
-  double ra[4], rb[4];
-  adj_load(g, 0, i0, c0, ra, rb);
+  T ra[4], rb[4];
+  adj_load<T>(g, 0, i0, c0, ra, rb);
   store_slice(As[0], ra);
   store_slice(Bs[0], rb);
   __syncthreads();
   for (int st = 0; st < steps; ++st) {
     const int cur = st & 1;
-    if (st + 1 < steps) adj_load(g, st + 1, i0, c0, ra, rb);
+    if (st + 1 < steps) adj_load<T>(g, st + 1, i0, c0, ra, rb);
     mma_slice(As[cur], Bs[cur], wr, wc, acc);
     if (st + 1 < steps) {
       store_slice(As[cur ^ 1], ra);
@@ -492,9 +499,9 @@ __global__ __launch_bounds__(256, 2) void k_adj_grad(AdjArgs g) {
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = i0 + wr * 32 + ti * 16 + out_row<double>(lg, r);
+        const int m = i0 + wr * 32 + ti * 16 + out_row<T>(lg, r);
         const int c = c0 + wc * 32 + tj * 16 + lr;
-        g.grad[(size_t)m * g.npd + c] += acc[ti][tj][r];
+        g.grad[(size_t)m * g.npd + c] += (double)acc[ti][tj][r];
       }
 }
 
@@ -541,6 +548,15 @@ void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *
   hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
 }
 
+void launch_gemm1_raw_f32(const Ensemble &e, int sb, const float *M, const double *G,
+                          double *out, hipStream_t st) {
+  Gemm1Args<float> g;
+  g.A = e.X_f; g.Ys = M;
+  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = G; g.W = out;
+  g.Np = e.Np; g.npd = e.npd; g.do_q = 1; g.raw = 1; g.kf = g.kr = g.ky = 1.0;
+  hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
+}
+
 void launch_gemm2(const Ensemble &e, int sb, int prec, hipStream_t st) {
   if (prec == kBF16x3) {
     launch_gemm2_bf16x3(e, sb, st);
@@ -564,12 +580,23 @@ void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
                      e.V, n_chunks, count);
 }
 
-void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, hipStream_t st) {
-  AdjArgs g;
+void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, int prec,
+                     hipStream_t st) {
+  const dim3 grid(e.Np / BM, e.npd / BN);
+  if (prec == kFP32) {
+    AdjArgs<float> g;
+    g.X = e.X_f; g.Us = e.Us_f32; g.Ys = e.Ys_f32;
+    g.W = e.W; g.Wu = e.Wu; g.G = e.G; g.Gu = e.Gu;
+    g.rho = e.rho; g.fups = e.fups; g.fpsi = e.fpsi; g.grad = e.gradp;
+    g.Np = e.Np; g.npd = e.npd; g.n = e.n; g.sb = sb; g.kf = kf; g.kr = kr; g.ky = ky;
+    hipLaunchKernelGGL(k_adj_grad<float>, grid, dim3(256), 0, st, g);
+    return;
+  }
+  AdjArgs<double> g;
   g.X = e.X; g.Us = e.Us; g.Ys = e.Ys; g.W = e.W; g.Wu = e.Wu; g.G = e.G; g.Gu = e.Gu;
   g.rho = e.rho; g.fups = e.fups; g.fpsi = e.fpsi; g.grad = e.gradp;
   g.Np = e.Np; g.npd = e.npd; g.n = e.n; g.sb = sb; g.kf = kf; g.kr = kr; g.ky = ky;
-  hipLaunchKernelGGL(k_adj_grad, dim3(e.Np / BM, e.npd / BN), dim3(256), 0, st, g);
+  hipLaunchKernelGGL(k_adj_grad<double>, grid, dim3(256), 0, st, g);
 }
 
 void launch_grad_out(const Ensemble &e, double *grad, int accumulate, hipStream_t st) {

@@ -754,7 +754,15 @@ int ensure_adjoint(tdscha_ensemble *h, int nc) {
     rc |= dev_alloc(h, &e.Us, sb * nn);
     rc |= dev_alloc(h, &e.Gu, sb * 2 * e.npd);
     rc |= dev_alloc(h, &e.Wu, sb * 4 * (size_t)e.Np);
+    e.Us_f32 = nullptr;
+    if (e.prec_mask & (1 << kFP32)) rc |= dev_alloc(h, &e.Us_f32, sb * nn);
     if (rc) return rc;
+    // the rotations write n x n of each slot: the padding the contractions read stays zero
+    if (hipMemset(e.Us, 0, sb * nn * 8) != hipSuccess ||
+        (e.Us_f32 && hipMemset(e.Us_f32, 0, sb * nn * 4) != hipSuccess)) {
+      set_error("adjoint workspace initialisation failed");
+      return 10;
+    }
     e.adj_slots = e.s_batch;
   }
   if (!e.gradp) {
@@ -778,10 +786,18 @@ int ensure_adjoint(tdscha_ensemble *h, int nc) {
 
 }  // namespace
 
-int tdscha_ensemble_force_grad(tdscha_ensemble *h, int n_pairs, const int *flags,
+int tdscha_ensemble_force_grad(tdscha_ensemble *h, int precision, int n_pairs, const int *flags,
                                const double *const *u, const double *const *v, double *grad,
                                int accumulate, void *stream) {
   if (!h || !flags || !u || !v || !grad) { set_error("NULL argument"); return 1; }
+  if (precision != kFP64 && precision != kFP32) {
+    set_error("the force adjoint runs in FP64 or FP32 (TDSCHA_PREC_FP64 / TDSCHA_PREC_FP32)");
+    return 4;
+  }
+  {
+    const int pc = check_prec(h->e, precision);
+    if (pc) return pc;
+  }
   if (n_pairs < 1 || n_pairs > TDSCHA_MAX_CHAINS) {
     set_error("n_pairs must be in [1, " + std::to_string(TDSCHA_MAX_CHAINS) + "]");
     return 1;
@@ -813,13 +829,19 @@ int tdscha_ensemble_force_grad(tdscha_ensemble *h, int n_pairs, const int *flags
     const int sb = ps.slots();
     // forward quantities of the images: Ys, (g1, g2), weights c1, c2, e1, e2
     launch_rot_sym(e, ps, e.Ysym, e.Ys, nullptr, st);
+    if (precision == kFP32) launch_split_ys(e, sb, kFP32, st);
     launch_rot_vec(e, ps, st);
-    launch_gemm1(e, sb, kFP64, what, st);
+    launch_gemm1(e, sb, precision, what, st);
     // seeds on the images: f_ups S^T U S f_ups, (f_ups S^T uR, S^T uR), then mu, nu
     launch_rot_sym(e, ps, e.Uc, e.Us, e.fups, st);
     launch_rot_vec_from(e, ps, e.uRc, e.Gu, e.ones, st);
-    launch_gemm1_raw(e, sb, e.Us, e.Gu, e.Wu, st);
-    launch_adj_grad(e, sb, 1.0, 1.0, ky, st);
+    if (precision == kFP32) {
+      launch_convert(e.Us, e.Us_f32, (size_t)sb * e.npd * e.npd, kFP32, st);
+      launch_gemm1_raw_f32(e, sb, e.Us_f32, e.Gu, e.Wu, st);
+    } else {
+      launch_gemm1_raw(e, sb, e.Us, e.Gu, e.Wu, st);
+    }
+    launch_adj_grad(e, sb, 1.0, 1.0, ky, precision, st);
   }
   launch_grad_out(e, grad, accumulate, st);
   TD_CHECK(hipGetLastError());

@@ -129,6 +129,7 @@ struct Ensemble {
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
+  float *Us_f32 = nullptr;    // the same in FP32 (FP32 adjoint)
   double *Gu = nullptr;       // [s_batch x 2 x npd]   f_ups S^T uR, S^T uR
   double *Wu = nullptr;       // [s_batch x 4 x Np]    mu, -, nu, - per image
   double *ones = nullptr;     // [npd]
@@ -184,7 +185,11 @@ void launch_post_adj(const Ensemble &e, const double *u, double *U, double *uR, 
 void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *G, double *out,
                       hipStream_t st);
 // ---- force adjoint (gemm.hip) ----
-void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, hipStream_t st);
+void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, int prec,
+                     hipStream_t st);
+// launch_gemm1_raw on FP32 copies (X_f and M in FP32, FP32 accumulation, FP64 row sums)
+void launch_gemm1_raw_f32(const Ensemble &e, int sb, const float *M, const double *G,
+                          double *out, hipStream_t st);
 void launch_grad_out(const Ensemble &e, double *grad, int accumulate, hipStream_t st);
 // BF16x3 forms (gemm_bf16.hip)
 int bf16_col_tiles(const Ensemble &e);

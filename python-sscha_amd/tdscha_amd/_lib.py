@@ -56,8 +56,8 @@ SIGNATURES = {
     "tdscha_ensemble_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "tdscha_ensemble_get_tuning": (ctypes.c_int, [_vp, _ip, _ip]),
     "tdscha_ensemble_force_grad": (ctypes.c_int, [
-        _vp, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, ctypes.c_int,
-        _vp]),
+        _vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
+        ctypes.c_int, _vp]),
     "tdscha_lanczos_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), _vp, ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, _dp]),

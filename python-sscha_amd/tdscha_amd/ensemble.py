@@ -227,9 +227,11 @@ class DeviceEnsemble:
                 "tdscha_ensemble_get_tuning")
         return v.value, s.value
 
-    def force_grad(self, us, vs, flags=None, out=None, accumulate=False):
+    def force_grad(self, us, vs, flags=None, out=None, accumulate=False, precision="fp64"):
         """d/dY of sum_p u_p . A_p(v_p) (tdscha_ensemble_force_grad): A_p is the anharmonic
-        part of the operator with flags[p]. Returns an (N, n) FP64 device tensor."""
+        part of the operator with flags[p]. Returns an (N, n) FP64 device tensor.
+        precision "fp64", or "fp32" (FP32 MFMA contractions with FP32 accumulation; needs an
+        ensemble created with the fp32 copies)."""
         t = self.torch
         n = self.n
         C = len(us)
@@ -241,12 +243,15 @@ class DeviceEnsemble:
         if out is None:
             out = t.zeros((self.N, n), dtype=t.float64, device=self.device)
             accumulate = False
-        elif tuple(out.shape) != (self.N, n) or out.dtype != t.float64 or not out.is_contiguous():
-            raise ValueError("out must be a contiguous float64 (N, n) tensor")
+        else:
+            self._check_out(out, (self.N, n))
         fl = (ctypes.c_int * C)(*[int(f) for f in flags])
         pu = (ctypes.c_void_p * C)(*[u.data_ptr() for u in us])
         pv = (ctypes.c_void_p * C)(*[v.data_ptr() for v in vs])
-        rc = L.lib().tdscha_ensemble_force_grad(self._h, C, fl, pu, pv, out.data_ptr(),
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("the force adjoint runs in fp64 or fp32, not %r" % (precision,))
+        rc = L.lib().tdscha_ensemble_force_grad(self._h, L.PRECISIONS[precision], C, fl, pu, pv,
+                                                out.data_ptr(),
                                                 int(bool(accumulate)),
                                                 _stream_handle(t, self.device))
         L.check(rc, "tdscha_ensemble_force_grad")

@@ -267,7 +267,7 @@ class GammaSectorOperator:
     def N(self):
         return self.ens.N
 
-    def force_grad(self, us, vs, flags=None, out=None, accumulate=False):
+    def force_grad(self, us, vs, flags=None, out=None, accumulate=False, precision="fp64"):
         """d/dY sum_p u_p . A_p(v_p) for v_p in the sector: A_p = P_f A_P, so
         u . A_p(v) = (P_f^T u) . A_P(v), the representatives' adjoint with u projected by the
         transposed projector (P0 <-> P0^T in the packed metric)."""
@@ -276,7 +276,8 @@ class GammaSectorOperator:
         n_psi = self.n + self.n * (self.n + 1)
         us = [self.proj.project(self.ens._dev(u, (n_psi,)), adjoint=not self._adjoint(f))
               for u, f in zip(us, flags)]
-        return self.ens.force_grad(us, vs, flags, out=out, accumulate=accumulate)
+        return self.ens.force_grad(us, vs, flags, out=out, accumulate=accumulate,
+                                   precision=precision)
 
     def autotune(self, precision="bf16x3", n_chains=2):
         return self.ens.autotune(precision, n_chains)

@@ -25,7 +25,7 @@ basis alive between the two.
 """
 import numpy as np
 
-from .DynamicalLanczos import Lanczos, MODE_GPU_FP64
+from .DynamicalLanczos import Lanczos, MODE_GPU_FP32, MODE_GPU_FP64
 
 
 class SpectralProblem:
@@ -33,10 +33,12 @@ class SpectralProblem:
     frequencies w_k (Ry), the smearing (Ry), the number of Lanczos steps and the mode."""
 
     def __init__(self, X, w, rho, T, sym, omegas, smearing, n_steps, mode=MODE_GPU_FP64,
-                 full_gs=True, gamma_sector=None, verbose=False):
+                 full_gs=True, gamma_sector=None, verbose=False, adjoint_precision=None):
         """gamma_sector: (rotations, irt, pols) of the supercell's space group to run the
         Lanczos and the adjoint on the zone-centre operator (tdscha_amd.gamma), for
-        translation-invariant psi0 such as IR perturbations."""
+        translation-invariant psi0 such as IR perturbations. adjoint_precision: "fp64" or
+        "fp32" for the force adjoint's contractions (default: "fp32" with MODE_GPU_FP32,
+        else "fp64")."""
         self.X = np.ascontiguousarray(X, dtype=np.float64)
         self.w = np.ascontiguousarray(w, dtype=np.float64)
         self.rho = np.ascontiguousarray(rho, dtype=np.float64)
@@ -49,6 +51,11 @@ class SpectralProblem:
         self.full_gs = bool(full_gs)
         self.gamma_sector = gamma_sector
         self.verbose = bool(verbose)
+        if adjoint_precision is None:
+            adjoint_precision = "fp32" if mode == MODE_GPU_FP32 else "fp64"
+        if adjoint_precision == "fp32" and mode != MODE_GPU_FP32:
+            raise ValueError("the FP32 adjoint needs the FP32 operand copies: mode=MODE_GPU_FP32")
+        self.adjoint_precision = adjoint_precision
 
     def lanczos(self, psi0, Y):
         lz = Lanczos(mode=self.mode)
@@ -132,7 +139,7 @@ def spectral_gradients(problem, psi0, Y, pairs_per_call=32):
     for j0 in range(0, k, pairs_per_call):
         j1 = min(k, j0 + pairs_per_call)
         ens.force_grad([U[j] for j in range(j0, j1)], [Q[j] for j in range(j0, j1)],
-                       out=gY, accumulate=True)
+                       out=gY, accumulate=True, precision=problem.adjoint_precision)
         if problem.verbose:
             torch.cuda.synchronize()
             print("force adjoint: pairs %d-%d of %d done" % (j0, j1 - 1, k), flush=True)

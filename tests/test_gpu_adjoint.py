@@ -93,3 +93,54 @@ def test_euler_identity_on_the_device_operator():
     g = dev.force_grad([u], [v], [0]).cpu().numpy()
     J = u.dot(Av)
     assert abs(J - np.sum(g * ens.Y)) < 1e-11 * np.sum(np.abs(g * ens.Y))
+
+
+def _dev_ens32(ens, s_batch=0):
+    from tdscha_amd.ensemble import DeviceEnsemble, SymmetryBlocks
+    s = ens.sym
+    return DeviceEnsemble(ens.X, ens.Y, ens.w, ens.rho, ens.T,
+                          SymmetryBlocks(s.symmetries, s.N_degeneracy, s.degenerate_space,
+                                         s.block_id), precisions=("fp64", "fp32"),
+                          s_batch=s_batch)
+
+
+# FP32 adjoint: the three contractions of a pass in FP32 MFMA with FP32 accumulation (the
+# operands carry 2^-24 relative rounding; X, the rotated seeds and the rotated perturbation
+# enter products of ~n terms): measured against the FP64 oracle adjoint, relative to its
+# largest entry.
+TOL32 = 2e-5
+
+
+@pytest.mark.parametrize("name", ["std", "transpose", "wigner", "reverse", "no_v4"])
+def test_fp32_force_grad_matches_oracle(name):
+    ens = fixtures.random_ensemble(81, 20, 30, 4)
+    rng = np.random.default_rng(82)
+    u, v = (rng.standard_normal(orc.psi_len(20)) for _ in range(2))
+    fl = _flags(name)
+    ref = _oracle_grad(ens, u, v, fl)
+    dev = _dev_ens32(ens)
+    g = dev.force_grad([u], [v], [fl], precision="fp32").cpu().numpy()
+    g64 = dev.force_grad([u], [v], [fl]).cpu().numpy()
+    err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+    err64 = np.max(np.abs(g64 - ref)) / np.max(np.abs(ref))
+    print("%s: fp32 %.2e, fp64 %.2e" % (name, err, err64))
+    assert err < TOL32 and err64 < TOL
+
+
+@pytest.mark.parametrize("max_block,seed", [(32, 84), (90, 86)])
+def test_fp32_force_grad_block_widths(max_block, seed):
+    ens = fixtures.random_ensemble(seed, 100, 40, 3, max_block=max_block)
+    rng = np.random.default_rng(seed + 1)
+    u, v = (rng.standard_normal(orc.psi_len(100)) for _ in range(2))
+    ref = _oracle_grad(ens, u, v, 0)
+    g = _dev_ens32(ens).force_grad([u], [v], [0], precision="fp32").cpu().numpy()
+    err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+    print("max_block %d: fp32 %.2e" % (max_block, err))
+    assert err < TOL32
+
+
+def test_fp32_force_grad_needs_the_fp32_copies():
+    ens = fixtures.random_ensemble(87, 12, 10, 2)
+    u = np.ones(orc.psi_len(12))
+    with pytest.raises(ValueError):
+        _dev_ens(ens).force_grad([u], [u], [0], precision="fp32")

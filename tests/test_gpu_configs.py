@@ -22,7 +22,9 @@ tdscha_amd.synthetic: the crystal, force model and sampling of the workload defi
       three systems: two steps of the group equal two steps of each member run alone, bit
       for bit.
   C5  4x4x4, 381 modes, 10000 configurations: gradient of sum_k Im G(w_k) over 64 points in
-      50-250 cm^-1 after 150 steps (zone-centre operator) against central differences.
+      50-250 cm^-1 after 150 steps (zone-centre operator) against central differences, in
+      FP64 and with FP32 contractions and FP32 accumulation (forward and adjoint) against the
+      FP64 gradient.
 The Lanczos tests scale the model's cubic and quartic terms by 20: with the benchmark model
 the IR vector couples to the two-phonon sector below the reference's absolute stop
 threshold |b| < 1e-12 (Py:6144-6153) and the recurrence stops after one step.
@@ -333,3 +335,49 @@ def spectral_value(prob, psi0, Y):
     J = prob.value(lz)
     lz.device_ensemble.close()
     return J
+
+
+def test_c5_fp32_gradients_against_fp64(c2_anh):
+    """The C5 gradients with FP32 MFMA contractions and FP32 accumulation in the forward
+    Lanczos and in the force adjoint (MODE_GPU_FP32), against the FP64 run. The recurrence
+    amplifies any operator change over 150 steps (between two FP64 summation orders: J
+    7.6e-7, dJ/dY 0.02 %, dJ/dpsi0 3.8 % of the largest entry), so the FP32 run is held to
+    J within 1e-4, and to directional derivatives along the FP64 gradients (psi0 and Y)
+    within 5 %, which its own central differences must also match."""
+    from tdscha_amd import DynamicalLanczos as DL
+    from tdscha_amd import synthetic as sy
+    from tdscha_amd.gradients import SpectralProblem, spectral_gradients
+    print("case: C5 gradients, FP32 forward and adjoint against FP64", flush=True)
+    e = c2_anh
+    n = len(e.w)
+    rots, irt = sy.space_group(e.crystal)
+    omegas = np.linspace(50.0, 250.0, 64) / sy.RY_TO_CM
+    psi0 = np.zeros(orc.psi_len(n))
+    psi0[:n] = e.ir_perturbation(0)
+    res = {}
+    probs = {}
+    for name, mode in (("fp64", DL.MODE_GPU_FP64), ("fp32", DL.MODE_GPU_FP32)):
+        probs[name] = SpectralProblem(e.X, e.w, e.rho, e.T, _sym(e), omegas, 1.0 / sy.RY_TO_CM,
+                                      150, mode=mode, gamma_sector=(rots, irt, e.pols))
+        res[name] = spectral_gradients(probs[name], psi0, e.Y)
+        print("%s: J = %.8e" % (name, res[name][0]), flush=True)
+    J0, gp0, gy0 = res["fp64"]
+    J1, gp1, gy1 = res["fp32"]
+    dJ = abs(J1 - J0) / abs(J0)
+    up = gp0 / np.linalg.norm(gp0)
+    uy = gy0 / np.linalg.norm(gy0)
+    dp = abs(gp1 @ up - gp0 @ up) / abs(gp0 @ up)
+    dy = abs(np.sum(gy1 * uy) - np.sum(gy0 * uy)) / abs(np.sum(gy0 * uy))
+    ey = np.abs(gy1 - gy0).max() / np.abs(gy0).max()
+    print("FP32 against FP64: J %.2e, d/dpsi0 along the FP64 gradient %.2e, d/dY along it "
+          "%.2e, max |dJ/dY| difference %.2e of the largest entry" % (dJ, dp, dy, ey))
+    assert dJ < 1e-4 and dp < 5e-2 and dy < 5e-2
+    # the FP32 gradient against the FP32 run's own central differences along uY
+    h = 1e-4
+    dY = uy * np.linalg.norm(e.Y)
+    fd = (spectral_value(probs["fp32"], psi0, e.Y + h * dY) -
+          spectral_value(probs["fp32"], psi0, e.Y - h * dY)) / (2 * h)
+    adj = float(np.sum(gy1 * dY))
+    print("FP32 Y direction: adjoint %.6e, central difference %.6e, relative %.2e"
+          % (adj, fd, abs(fd - adj) / abs(fd)))
+    assert abs(fd - adj) / abs(fd) < 5e-2
