@@ -18,9 +18,12 @@
  *   degenerate_space[b][0..d_b-1], which must be contiguous and sorted
  *   (the reference indexes rows as mu - degenerate_space[b][0],
  *   LanczosFunctions.c:2162), blocks_ids[mu] = block of mode mu.
- * Frequencies must be equal (to 1e-8 relative) inside a block, as they are for
- * blocks of degenerate modes; the device path uses this to rotate the n x n
- * operands instead of building every image.
+ * When the frequencies are equal (to 1e-8 relative) inside every block, as they are for
+ * blocks of degenerate modes, the device path rotates the n x n operands instead of
+ * building every image. An ensemble with a block of unequal frequencies is accepted, as
+ * the reference accepts it: it runs on an exact image path (the images of every
+ * configuration built on the device, one symmetry at a time), in FP64 only; the reduced
+ * precisions and the force adjoint refuse it.
  *
  * Thread safety: calls on different ensembles may run concurrently; calls on
  * one ensemble share its workspace and must not overlap.

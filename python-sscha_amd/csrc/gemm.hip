@@ -580,6 +580,34 @@ void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
                      e.V, n_chunks, count);
 }
 
+// ---- the exact image path (images.hip): FP64 contractions of one image slot
+void launch_gemm1_images(const Ensemble &e, const double *Ysym_c, int what, hipStream_t st) {
+  Gemm1Args<double> g;
+  g.A = e.IU; g.Ys = Ysym_c;
+  g.X = e.IU; g.Ft = e.IFt; g.rho = e.rho; g.G = e.G; g.W = e.W;
+  g.Np = e.Np; g.npd = e.npd;
+  g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  g.raw = 0;
+  g.kf = (what & kTermF) ? 1.0 : 0.0;
+  g.kr = (what & kTermD2vR) ? 1.0 : 0.0;
+  g.ky = (what & kTermD2vY) ? 1.0 : 0.0;
+  hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, 1), dim3(256), 0, st, g);
+}
+
+void launch_gemm2_images(const Ensemble &e, hipStream_t st) {
+  Gemm2Args<double> g;
+  g.XtT = e.IUT; g.YT = e.IFT; g.W = e.W; g.B = e.B; g.Np = e.Np; g.npd = e.npd;
+  hipLaunchKernelGGL(k_gemm2<double>, dim3(e.npd / BM, e.npd / BN, 1), dim3(256), 0, st, g);
+}
+
+void launch_fav_images(const Ensemble &e, hipStream_t st) {
+  const int n_chunks = (e.Np + kFavChunk - 1) / kFavChunk;
+  dim3 grid(e.npd / BM, n_chunks, 1);
+  hipLaunchKernelGGL(k_fav_mfma, grid, dim3(256), 0, st, e.IFT, e.IUT, e.W, e.Vp, e.Np, e.npd, 1);
+  hipLaunchKernelGGL(k_fav_reduce, dim3((unsigned)((e.npd + 255) / 256)), dim3(256), 0, st, e.Vp,
+                     e.V, n_chunks, (size_t)e.npd);
+}
+
 void launch_adj_grad(const Ensemble &e, int sb, double kf, double kr, double ky, int prec,
                      hipStream_t st) {
   const dim3 grid(e.Np / BM, e.npd / BN);

@@ -103,12 +103,9 @@ int check_blocks(int n, int n_blocks, const int *N_deg, const int *const *deg_sp
         set_error("blocks_ids[" + std::to_string(m) + "] disagrees with degenerate_space");
         return 2;
       }
-      double rel = std::fabs(w[m] - w[m0]) / std::fabs(w[m0]);
-      if (!(rel <= 1e-8)) {
-        set_error("frequencies inside symmetry block " + std::to_string(b) +
-                  " differ by more than 1e-8 relative: blocks must hold degenerate modes");
-        return 3;
-      }
+      // blocks of modes with different frequencies take the exact image path (FP64)
+      const double rel = std::fabs(w[m] - w[m0]) / std::fabs(w[m0]);
+      if (!(rel <= 1e-8)) e.unequal = true;
     }
     e.blk_off[b] = m0;
     e.blk_dim[b] = d;
@@ -195,9 +192,45 @@ int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st, in
   const bool need_r = what & kTermD2vR;
   const bool need_d = what & (kTermD2vR | kTermD2vY);
   const bool need_f = what & kTermF;
+  if (e.unequal && prec != kFP64) {
+    set_error("this ensemble has symmetry blocks whose modes differ in frequency (by more than "
+              "1e-8 relative): it runs on the exact image path, in FP64 only");
+    return 4;
+  }
   launch_zero(e.f_acc, (size_t)nc * e.npd, st);
   launch_zero(e.d2v_acc, (size_t)nc * e.npd * e.npd, st);
   if (!need_r) launch_zero(e.G, (size_t)e.s_batch * 2 * e.npd, st);
+  if (e.unequal) {
+    // the images of every configuration, one symmetry at a time, shared by the chains
+    for (int s = s_begin; s < s_end; ++s) {
+      {
+        Span sp(TDSCHA_KGROUP_ROTATE, st);
+        launch_images(e, s, st);
+      }
+      for (int c = 0; c < nc; ++c) {
+        if (need_r) launch_image_g(e, c, st);
+        {
+          Span sp(TDSCHA_KGROUP_GEMM1, st);
+          launch_gemm1_images(e, e.Ysym + (size_t)c * e.npd * e.npd, what, st);
+        }
+        if (need_d) {
+          Span sp(TDSCHA_KGROUP_GEMM2, st);
+          launch_gemm2_images(e, st);
+        }
+        if (need_f) {
+          Span sp(TDSCHA_KGROUP_FAV, st);
+          launch_fav_images(e, st);
+        }
+        launch_image_acc(e, c, need_d, need_f, st);
+      }
+    }
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+      set_error(std::string("kernel launch failed: ") + hipGetErrorString(err));
+      return 12;
+    }
+    return 0;
+  }
   const int slots = (e.pass_slots > 0 && e.pass_slots < e.s_batch) ? e.pass_slots : e.s_batch;
   const int ns_max = slots / nc > 0 ? slots / nc : 1;
   for (int s0 = s_begin; s0 < s_end; s0 += ns_max) {
@@ -482,6 +515,13 @@ int ensemble_create_impl(tdscha_ensemble **out, int device, const double *X, con
   e.n_pairs = (int)hpairs.size();
   rc |= dev_alloc(h, &e.dgroups, hgroups.size());
   rc |= dev_alloc(h, &e.dpairs, hpairs.size());
+  if (e.unequal) {
+    const size_t NXi = (size_t)Np * npd;
+    rc |= dev_alloc(h, &e.IU, NXi);
+    rc |= dev_alloc(h, &e.IFt, NXi);
+    rc |= dev_alloc(h, &e.IUT, NXi);
+    rc |= dev_alloc(h, &e.IFT, NXi);
+  }
   const bool fixed_overflow = h->arena_overflow;   // before any per-slot buffer
   if (!rc) rc |= alloc_slots(h, e.s_batch);
   if (!rc) rc |= ensure_chains(h, 1);
@@ -797,6 +837,11 @@ int tdscha_ensemble_force_grad(tdscha_ensemble *h, int precision, int n_pairs, c
   {
     const int pc = check_prec(h->e, precision);
     if (pc) return pc;
+  }
+  if (h->e.unequal) {
+    set_error("the force adjoint needs symmetry blocks of degenerate modes (equal frequencies "
+              "to 1e-8 relative); this ensemble runs on the exact image path");
+    return 3;
   }
   if (n_pairs < 1 || n_pairs > TDSCHA_MAX_CHAINS) {
     set_error("n_pairs must be in [1, " + std::to_string(TDSCHA_MAX_CHAINS) + "]");

@@ -133,6 +133,10 @@ struct Ensemble {
   double *Gu = nullptr;       // [s_batch x 2 x npd]   f_ups S^T uR, S^T uR
   double *Wu = nullptr;       // [s_batch x 4 x Np]    mu, -, nu, - per image
   double *ones = nullptr;     // [npd]
+  // exact image path (a block holds modes of different frequencies): the images of one slot
+  bool unequal = false;
+  double *IU = nullptr, *IFt = nullptr;   // [Np x npd]  u = S x, f~ = f_psi S y
+  double *IUT = nullptr, *IFT = nullptr;  // [npd x Np]  u~^T = (f_ups S x)^T, f^T = (S y)^T
   double *gradp = nullptr;    // [Np x npd]            dJ/dY, padded
 };
 
@@ -170,6 +174,14 @@ void launch_finalize(const Ensemble &e, int what, bool d2v_accumulate, double *f
 void launch_split_ys(const Ensemble &e, int sb, int prec, hipStream_t st);
 // fp64 -> bf16 hi/lo planes (dst, dst + count) or fp32
 void launch_convert(const double *src, void *dst, size_t count, int prec, hipStream_t st);
+
+// ---- exact image path (images.hip; contractions in gemm.hip) ----
+void launch_images(const Ensemble &e, int s, hipStream_t st);
+void launch_image_g(const Ensemble &e, int chain, hipStream_t st);
+void launch_image_acc(const Ensemble &e, int chain, bool do_d, bool do_f, hipStream_t st);
+void launch_gemm1_images(const Ensemble &e, const double *Ysym_c, int what, hipStream_t st);
+void launch_gemm2_images(const Ensemble &e, hipStream_t st);
+void launch_fav_images(const Ensemble &e, hipStream_t st);
 
 // ---- operator pre/post transforms and harmonic part (operator.hip) ----
 void launch_unpack(const Ensemble &e, int chain, const double *psi, int flags, hipStream_t st);

@@ -68,3 +68,13 @@ def random_perturbation(seed, n, symmetric=True):
     A = rng.standard_normal((n, n)) * 1e-4
     Y1 = 0.5 * (A + A.T) if symmetric else A
     return R1, Y1
+
+
+def split_frequencies(ens, rel):
+    """The same ensemble with the frequencies inside every block of more than one mode
+    spread by up to `rel` relative (a block of modes that are not exactly degenerate)."""
+    w = ens.w.copy()
+    for d in ens.sym.degenerate_space:
+        if len(d) > 1:
+            w[d] *= 1 + rel * np.linspace(-1.0, 1.0, len(d))
+    return orc.Ensemble(ens.X, ens.Y, w, ens.rho, ens.T, ens.sym)

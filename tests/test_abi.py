@@ -62,16 +62,6 @@ def test_rejects_non_contiguous_blocks():
     assert rc == 2 and "blocks_ids" in msg or "contiguous" in msg
 
 
-def test_rejects_non_degenerate_block():
-    ens = fixtures.random_ensemble(2, 8, 4, 2, max_block=3)
-    big = int(np.argmax(ens.sym.N_degeneracy))
-    assert ens.sym.N_degeneracy[big] > 1
-    w = ens.w.copy()
-    w[ens.sym.degenerate_space[big][-1]] *= 1.01
-    rc, msg = _create(ens, w=w)
-    assert rc == 3 and "degenerate" in msg
-
-
 def test_rejects_nonpositive_frequency():
     ens = fixtures.random_ensemble(3, 5, 4, 2)
     w = ens.w.copy()

@@ -148,6 +148,38 @@ def test_precision_forms(precision, tol):
     _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
 
 
+@pytest.mark.parametrize("rel,d4", [(1e-6, 1), (0.05, 1), (0.05, 0)])
+def test_blocks_with_unequal_frequencies(rel, d4):
+    """Blocks whose modes differ in frequency (by 1e-6 and 5 % relative) take the exact image
+    path: the perturbed averages and the full L against the oracle (which the host tests pin
+    on the reference's own C build for such blocks), plain relative 1e-10."""
+    from tdscha_amd import _lib as L
+    ens = fixtures.split_frequencies(fixtures.random_ensemble(95, 23, 40, 5, max_block=6), rel)
+    print("case: split blocks, rel %g, d4 %d" % (rel, d4), flush=True)
+    R1, Y1 = fixtures.random_perturbation(96, 23)
+    dev = _dev_ens(ens)
+    _check(ens, R1, Y1, d4, dev=dev)
+    psi = np.random.default_rng(97).standard_normal(orc.psi_len(23))
+    for fl, kw in ((0, {}), (L.OP_TRANSPOSE, dict(transpose=True)),
+                   (L.OP_WIGNER, dict(use_wigner=True))):
+        out = dev.apply_full_L(psi, flags=fl).cpu().numpy()
+        ref = orc.apply_full_L(ens, psi, **kw)
+        err = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        print(kw, "full L relative error %.2e" % err)
+        assert err < TOL64
+
+
+def test_unequal_blocks_refuse_reduced_precision_and_the_adjoint():
+    ens = fixtures.split_frequencies(fixtures.random_ensemble(98, 12, 20, 3, max_block=4), 0.01)
+    dev = _dev_ens(ens, precisions=("fp64", "bf16x3"))
+    R1, Y1 = fixtures.random_perturbation(99, 12)
+    with pytest.raises(ValueError, match="FP64 only"):
+        dev.apply(R1, Y1, precision="bf16x3")
+    u = np.ones(orc.psi_len(12))
+    with pytest.raises(ValueError, match="degenerate"):
+        dev.force_grad([u], [u], [0])
+
+
 def test_zero_temperature_and_uniform_weights():
     ens = fixtures.random_ensemble(11, 30, 40, 4, T=0.0, weights="uniform")
     R1, Y1 = fixtures.random_perturbation(12, 30)

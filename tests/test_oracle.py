@@ -33,6 +33,25 @@ def test_oracle_matches_reference_build_random(seed, d4):
     assert _scaled(d, dr, da) < 1e-14
 
 
+@pytest.mark.skipif(not orc.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("rel,d4", [(1e-6, 1), (0.05, 1), (0.05, 0)])
+def test_oracle_matches_reference_build_with_split_blocks(rel, d4):
+    """Blocks whose modes differ in frequency: the reference applies f_ups / f_psi per mode
+    to each image (C:2214-2221, 2401-2407, 2708-2725), as the oracle does."""
+    ens = fixtures.split_frequencies(fixtures.random_ensemble(95, 23, 9, 5, max_block=6), rel)
+    R1, Y1 = fixtures.random_perturbation(96, 23)
+    f, d, fa, da = orc.perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, d4,
+                                           ens.sym, with_abs=True)
+    fr, dr = orc.ref_perturb_average_sym(ens.X, ens.Y, ens.w, ens.rho, R1, Y1, ens.T, d4, ens.sym)
+    assert _scaled(f, fr, fa) < 1e-14
+    assert _scaled(d, dr, da) < 1e-14
+    # and the split is visible: the result differs from the block-averaged frequencies'
+    eq = orc.Ensemble(ens.X, ens.Y, fixtures.random_ensemble(95, 23, 9, 5, max_block=6).w,
+                      ens.rho, ens.T, ens.sym)
+    f0, d0 = orc.perturb_average_sym(eq.X, eq.Y, eq.w, eq.rho, R1, Y1, eq.T, d4, eq.sym)
+    assert np.linalg.norm(d - d0) > 1e3 * 1e-14 * np.linalg.norm(da)
+
+
 def test_symmetry_expansion_equals_unwrapped_ensemble():
     """Sym-expanded average == plain average over the explicit images
     (tests/_heavy_tests/test_fast_lanczos_snse/test_fast_lanczos_snse.py:66-76)."""
