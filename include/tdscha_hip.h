@@ -261,6 +261,25 @@ int tdscha_lanczos_read_gs(tdscha_lanczos *lz, int chain, double *norms, int *st
 /* Device pointers to the stored basis of a chain ((max_steps + 1) x (n + n (n + 1))). */
 int tdscha_lanczos_basis(tdscha_lanczos *lz, int chain, const double **Q, const double **P);
 
+/* ---- Translation average (the zone-centre operator's projector) -------------------------
+ * No reference counterpart. For psi = (R (n), two packed symmetric n x n matrices) of a
+ * supercell whose pure translations (n_trans x nat atom permutations irt, host) act freely on
+ * the atoms, P0 psi = the average of psi over the translations (adjoint = 0), or P0^T psi in
+ * the packed metric (adjoint = 1: the projector of transposed standard-representation
+ * applications). pols: host (3 nat x n_modes) polarization vectors of the kept modes,
+ * orthonormal columns. Device vectors; out may be psi. A projector's workspace is shared by
+ * its calls: calls on one projector must not overlap. */
+typedef struct tdscha_projector tdscha_projector;
+int tdscha_projector_create(tdscha_projector **out, int device, int n_modes, int nat,
+                            const double *pols, int n_trans, const int *irt);
+int tdscha_projector_destroy(tdscha_projector *p);
+int tdscha_projector_apply(tdscha_projector *p, int adjoint, const double *psi, double *out,
+                           void *stream);
+/* Run a Lanczos on the zone-centre operator: its ensemble holds one symmetry per rotation
+ * and every L application of a step is followed by p (transposed for the L^T p of the
+ * standard representation), inside the captured step. Set before the first run. */
+int tdscha_lanczos_set_projector(tdscha_lanczos *lz, tdscha_projector *p);
+
 /* Several Lanczos runs on different ensembles (systems of uneven size) advanced together:
  * one graph per step holds the step of every member, each on a branch of its own, so the
  * kernels of a small system run beside those of a large one. Members stay usable on their

@@ -402,6 +402,9 @@ struct tdscha_lanczos {
   int full_gs = 0;
   double *pn_all = nullptr, *gs_part = nullptr, *gs_coef = nullptr, *gsn = nullptr;
   long host_steps = 0;   // steps enqueued since the last start
+  // zone-centre operator: the translation average applied to every L q (and, transposed in
+  // the packed metric for the standard representation, to every L^T p) inside the step
+  tdscha_projector *proj = nullptr;
   std::vector<const double *> apply_in;
   std::vector<double *> apply_out;
   std::vector<int> apply_flags;
@@ -432,6 +435,14 @@ int enqueue_step(tdscha_lanczos *z, hipStream_t st) {
                                        z->apply_shifts.data(), z->apply_in.data(),
                                        z->apply_out.data(), st);
   if (rc) return rc;
+  if (z->proj) {
+    for (int k = 0; k < napply; ++k) {
+      const int fl = z->apply_flags[k];
+      const int adjoint = (fl & TDSCHA_OP_TRANSPOSE) && !(fl & TDSCHA_OP_WIGNER);
+      rc = projector_apply(z->proj, adjoint, z->apply_out[k], z->apply_out[k], st);
+      if (rc) return rc;
+    }
+  }
   // a = p_norm <p, L q>  (symmetric: <q, L q>_mask)
   hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks, C), dim3(256), 0, st,
                      (const double *const *)z->dpsi_p, (const double *const *)(z->dpsi_in + C),
@@ -575,6 +586,16 @@ int tdscha_lanczos_create(tdscha_lanczos **out, tdscha_ensemble *ens, int precis
   return 0;
 }
 
+int tdscha_lanczos_set_projector(tdscha_lanczos *z, tdscha_projector *p) {
+  if (!z) { set_error("NULL Lanczos"); return 1; }
+  if (z->exec) {
+    set_error("the projector must be set before the first graph is captured");
+    return 1;
+  }
+  z->proj = p;
+  return 0;
+}
+
 int tdscha_lanczos_destroy(tdscha_lanczos *z) {
   if (!z) return 0;
   if (z->exec) (void)hipGraphExecDestroy(z->exec);
@@ -708,6 +729,11 @@ int tdscha_lanczos_group_create(tdscha_lanczos_group **out, tdscha_lanczos *cons
       set_error("members of a group must be on one device");
       return 1;
     }
+    for (int j = 0; j < i; ++j)
+      if (members[i]->proj && members[j]->proj == members[i]->proj) {
+        set_error("members of a group must use different projectors (they share its workspace)");
+        return 1;
+      }
   }
   tdscha_lanczos_group *g = new (std::nothrow) tdscha_lanczos_group();
   if (!g) { set_error("out of host memory"); return 11; }

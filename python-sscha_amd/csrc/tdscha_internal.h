@@ -141,6 +141,15 @@ struct Ensemble {
 };
 
 void set_error(const std::string &msg);
+}  // namespace tdscha
+
+struct tdscha_projector;
+
+namespace tdscha {
+// P0 psi (adjoint = 0) or P0^T psi (adjoint = 1) on device vectors, enqueued on st; out may
+// be psi (projector.hip)
+int projector_apply(tdscha_projector *p, int adjoint, const double *psi, double *out,
+                    hipStream_t st);
 // turn the kernel-group event timing on / off (no event records may enter a captured
 // graph); returns the previous setting
 bool profile_swap(bool on);

@@ -886,11 +886,8 @@ class Lanczos(object):
 
     def _run_device_loop(self, n_iter, run_simm, graph, save_dir, prefix, full_gs=False):
         from .lanczos_device import DeviceLanczos
-        ens = self.device_ensemble
-        if self.gamma_sector:
-            raise NotImplementedError("device_loop=True runs the recurrence on the full "
-                                      "ensemble's graphs; use device_loop=False with "
-                                      "gamma_sector")
+        ens = self.device_ensemble     # a GammaSectorOperator with gamma_sector: the
+        #                               translation average runs inside the captured step
         if run_simm and not self.use_wigner:
             raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
         psi = np.asarray(self.psi, dtype=TYPE_DP)

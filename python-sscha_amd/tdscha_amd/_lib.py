@@ -63,6 +63,11 @@ SIGNATURES = {
         ctypes.c_int, ctypes.c_int, _dp]),
     "tdscha_lanczos_read_gs": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _ip]),
     "tdscha_lanczos_destroy": (ctypes.c_int, [_vp]),
+    "tdscha_lanczos_set_projector": (ctypes.c_int, [_vp, _vp]),
+    "tdscha_projector_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, _dp, ctypes.c_int, _ip]),
+    "tdscha_projector_destroy": (ctypes.c_int, [_vp]),
+    "tdscha_projector_apply": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
     "tdscha_lanczos_start": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), _vp]),
     "tdscha_lanczos_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "tdscha_lanczos_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),

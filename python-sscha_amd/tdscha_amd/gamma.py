@@ -18,13 +18,15 @@ at the zone centre are in that sector and L keeps them there, so their Lanczos c
 P0 runs in Cartesian space, where a translation is a permutation of the atoms:
 S_t M S_t^T = E^T P_t (E M E^T) P_t^T E for the polarization vectors E of the kept modes; the
 average over the translations of a Cartesian matrix depends only on the orbits of its two
-atoms and their relative translation (the translations commute and act freely), so it is one
-scatter over the atom pairs and one gather back. psi
+atoms and their relative translation (the translations commute and act freely), so it is a
+sum over the pairs of each (orbit, orbit, relative translation) key, written back to every
+pair of the key (the library's tdscha_projector_* kernels). psi
 holds upper triangles of symmetric matrices, whose dot product counts an off-diagonal pair
 once; in that metric the transpose of L is covariant with P0^T = W P0 W^-1 (W = 2 on the
 off-diagonal entries), which is the projector for a transposed application in the standard
 representation. In the Wigner representation both directions use P0.
 """
+import ctypes
 import warnings
 
 import numpy as np
@@ -75,7 +77,13 @@ class TranslationProjector:
     """P0 and its packed adjoint on device vectors psi = (R (n), two packed n x n upper
     triangles). pols: (3 nat, n) polarization vectors of the kept modes (orthonormal
     columns); irt_translations: (n_t, nat) the atom each pure translation carries every atom
-    to."""
+    to.
+
+    When the translations act freely on the atoms (as the pure translations of a supercell
+    do) the projection runs in the library's kernels (tdscha_projector_*: the orbit-folded
+    average, deterministic, four FP64 MFMA products per matrix), which the device Lanczos can
+    also run inside its captured step (``handle``). Otherwise it falls back to the explicit
+    average over every translation, on the device with torch."""
 
     def __init__(self, pols, irt_translations, device=0, chunk=16):
         torch = _torch()
@@ -91,58 +99,43 @@ class TranslationProjector:
         if np.any(np.sort(irt, axis=1) != np.arange(nat)[None]):
             raise ValueError("every row of irt_translations must be a permutation of the atoms")
         self.n = E.shape[1]
+        self.nat, self.nt = nat, len(irt)
+        self._h = None
+        if _orbit_fold(irt) is not None:
+            irt32 = np.ascontiguousarray(irt, dtype=np.intc)
+            h = ctypes.c_void_p()
+            L.check(L.lib().tdscha_projector_create(
+                ctypes.byref(h), self.dev.index or 0, self.n, nat, E.ctypes.data_as(L._dp),
+                len(irt), irt32.ctypes.data_as(L._ip)), "tdscha_projector_create")
+            self._h = h
+            return
+        warnings.warn("translation average: the translations do not act freely on the "
+                      "atoms; using the explicit average over every translation")
         self.E = torch.as_tensor(E, device=self.dev)
         cart = (3 * irt[:, :, None] + np.arange(3)[None, None]).reshape(len(irt), -1)
         self.perm = torch.as_tensor(cart, device=self.dev)           # (n_t, 3 nat)
         self.chunk = chunk
-        self.nat, self.nt = nat, len(irt)
-        self.fold = None
-        fold = _orbit_fold(irt)
-        if fold is not None:
-            s_orb, idx, nb = fold
-            self.s_orb = torch.as_tensor(s_orb, device=self.dev)
-            self.idx = torch.as_tensor(idx.ravel(), device=self.dev)
-            self.nb = nb
-            # self-check on a private generator (the caller's random stream is not touched)
-            gen = torch.Generator(device=self.dev).manual_seed(12345)
-            C = torch.randn(3 * nat, 3 * nat, dtype=torch.float64, device=self.dev,
-                            generator=gen)
-            r = torch.randn(3 * nat, dtype=torch.float64, device=self.dev, generator=gen)
-            self.fold = True
-            ok = (torch.allclose(self._avg_mat(C), self._avg_mat_gather(C), rtol=0, atol=1e-12)
-                  and torch.allclose(self._avg_vec(r), r[self.perm].mean(0), rtol=0, atol=1e-12))
-            self.fold = True if ok else None
-            if not ok:
-                warnings.warn("translation average: the orbit-folded form disagrees with the "
-                              "average over every translation; using the slower gather form")
-        else:
-            warnings.warn("translation average: the translations do not act freely on the "
-                          "atoms; using the gather form (O(n_t nat^2) per matrix)")
         n = self.n
         iu = np.triu_indices(n)
         self.iu = (torch.as_tensor(iu[0], device=self.dev), torch.as_tensor(iu[1], device=self.dev))
         w = np.where(iu[0] < iu[1], 2.0, 1.0)
         self.w_off = torch.as_tensor(w, device=self.dev)
 
-    # the average over the group is the same through a permutation or its inverse
-    def _avg_vec(self, c):
-        if self.fold is None:
-            return c[self.perm].mean(0)
-        V = self.torch.zeros((self.nb, 3), dtype=c.dtype, device=c.device)
-        V.index_add_(0, self.s_orb, c.view(self.nat, 3))
-        return (V[self.s_orb] / self.nt).reshape(-1)
+    @property
+    def handle(self):
+        """The library projector (None for the explicit-average fallback)."""
+        return self._h
 
-    def _avg_mat(self, C):
-        """(1/n_t) sum_t C[t(a)][t(b)] = G[s(a)][s(b)][tau_a^-1 tau_b]: one scatter over the
-        atom pairs into (orbit, orbit, relative translation) and one gather back."""
-        if self.fold is None:
-            return self._avg_mat_gather(C)
-        nat = self.nat
-        B = C.view(nat, 3, nat, 3).permute(0, 2, 1, 3).reshape(nat * nat, 9)
-        G = self.torch.zeros((self.nb * self.nb * self.nt, 9), dtype=C.dtype, device=C.device)
-        G.index_add_(0, self.idx, B)
-        A = (G[self.idx] / self.nt).view(nat, nat, 3, 3)
-        return A.permute(0, 2, 1, 3).reshape(3 * nat, 3 * nat)
+    def close(self):
+        if self._h is not None and self._h.value:
+            L.lib().tdscha_projector_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _avg_mat_gather(self, C):
         acc = self.torch.zeros_like(C)
@@ -151,12 +144,6 @@ class TranslationProjector:
             acc += C[p[:, :, None], p[:, None, :]].sum(0)
         return acc / self.perm.shape[0]
 
-    def vector(self, r):
-        return self.E.T @ self._avg_vec(self.E @ r)
-
-    def matrix(self, M):
-        return self.E.T @ self._avg_mat(self.E @ M @ self.E.T) @ self.E
-
     def project(self, psi, adjoint=False, out=None):
         """P0 psi, or P0^T psi (adjoint=True)."""
         torch = self.torch
@@ -164,9 +151,23 @@ class TranslationProjector:
         nw2 = n * (n + 1) // 2
         if psi.shape[0] != n + 2 * nw2:
             raise ValueError("psi has %d entries, expected %d" % (psi.shape[0], n + 2 * nw2))
+        if self._h is not None:
+            psi = psi.to(device=self.dev, dtype=torch.float64).contiguous()
+            if out is None:
+                out = torch.empty_like(psi)
+            elif (out.dtype != torch.float64 or out.device != psi.device or
+                  not out.is_contiguous() or out.shape != psi.shape):
+                raise ValueError("out must be a contiguous float64 tensor like psi")
+            st = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+            L.check(L.lib().tdscha_projector_apply(self._h, int(bool(adjoint)),
+                                                   ctypes.c_void_p(psi.data_ptr()),
+                                                   ctypes.c_void_p(out.data_ptr()), st),
+                    "tdscha_projector_apply")
+            return out
         if out is None:
             out = torch.empty_like(psi)
-        out[:n] = self.vector(psi[:n])
+        E = self.E
+        out[:n] = E.T @ (E @ psi[:n])[self.perm].mean(0)
         for k in range(2):
             v = psi[n + k * nw2:n + (k + 1) * nw2]
             if adjoint:
@@ -174,7 +175,7 @@ class TranslationProjector:
             M = torch.zeros((n, n), dtype=psi.dtype, device=psi.device)
             M[self.iu] = v
             M = M + M.triu(1).T
-            r = self.matrix(M)[self.iu]
+            r = (E.T @ self._avg_mat_gather(E @ M @ E.T) @ E)[self.iu]
             out[n + k * nw2:n + (k + 1) * nw2] = r * self.w_off if adjoint else r
         return out
 
@@ -290,6 +291,7 @@ class GammaSectorOperator:
 
     def close(self):
         self.ens.close()
+        self.proj.close()
 
 
 def _main():

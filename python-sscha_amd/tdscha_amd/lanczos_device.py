@@ -40,10 +40,20 @@ class DeviceLanczos:
 
     flags[c], shifts[c]: TDSCHA_OP_* bits (WIGNER, REVERSE, IGNORE_*) and shift of chain c.
     symmetric: the Wigner symmetric form (run_simm), needs mask (mask_dot_wigner).
+    dev_ens may be a gamma.GammaSectorOperator: the chains then run on the zone-centre
+    operator, its translation average inside every captured step (their starting vectors
+    must be translation-invariant; this is checked).
     """
 
     def __init__(self, dev_ens, n_chains=1, precision="fp64", flags=None, shifts=None,
                  symmetric=False, max_steps=100, keep_basis=False, mask=None, full_gs=False):
+        self.gamma = None
+        if hasattr(dev_ens, "proj") and hasattr(dev_ens, "ens"):
+            if dev_ens.proj.handle is None:
+                raise ValueError("the device Lanczos needs the library projector (translations "
+                                 "acting freely on the atoms)")
+            self.gamma = dev_ens
+            dev_ens = dev_ens.ens
         self.ens = dev_ens
         self.torch = dev_ens.torch
         self.nc = int(n_chains)
@@ -69,6 +79,10 @@ class DeviceLanczos:
                                            int(self.keep_basis), int(self.full_gs), m)
         L.check(rc, "tdscha_lanczos_create")
         self._h = h
+        self._flags = flags
+        if self.gamma is not None:
+            L.check(L.lib().tdscha_lanczos_set_projector(self._h, self.gamma.proj.handle),
+                    "tdscha_lanczos_set_projector")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -86,6 +100,13 @@ class DeviceLanczos:
         if len(psis) != self.nc:
             raise ValueError("expected %d starting vectors" % self.nc)
         self._psi0 = [self.ens._dev(p, (self.n_psi,)) for p in psis]
+        if self.gamma is not None:
+            for p, f in zip(self._psi0, self._flags):
+                for adj in ((False,) if self.symmetric or f & L.OP_WIGNER else (False, True)):
+                    r = self.gamma.proj.residual(p, adjoint=adj)
+                    if r > 1e-8:
+                        raise ValueError("the zone-centre Lanczos needs translation-invariant "
+                                         "starting vectors (|psi - P0 psi| / |psi| = %g)" % r)
         pp = (ctypes.c_void_p * self.nc)(*[p.data_ptr() for p in self._psi0])
         rc = L.lib().tdscha_lanczos_start(self._h, pp, _stream_handle(t, self.ens.device))
         L.check(rc, "tdscha_lanczos_start")

@@ -230,16 +230,17 @@ class SyntheticEnsemble:
 
     def raman_like_perturbation(self, component, seed=7):
         """R-sector perturbation of one polarizability component (0..5: xx, yy, zz, xy, xz,
-        yz) from seeded synthetic per-atom Raman tensors d(alpha_ij)/dR_a, in the form of
-        prepare_raman (Modules/DynamicalLanczos.py:1693-1781). Rock salt has no first-order
-        Raman activity, so these tensors are not symmetric: they only give twelve-chain
-        batches realistic, distinct starting vectors."""
-        nat = self.crystal.nat
+        yz) from seeded synthetic Raman tensors d(alpha_ij)/dR_a, one per atom of the
+        primitive cell and the same in every cell (a zone-centre perturbation, as the
+        reference's prepare_raman builds it from the unit cell's tensor,
+        Modules/DynamicalLanczos.py:1003-1124), in the form of Py:1693-1781. Rock salt has no
+        first-order Raman activity, so these tensors are not symmetric: they only give
+        twelve-chain batches realistic, distinct starting vectors."""
         pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
         i, j = pairs[component]
-        t = np.random.default_rng(seed).standard_normal((nat, 3, 3, 3))
+        t = np.random.default_rng(seed).standard_normal((2, 3, 3, 3))
         t = 0.5 * (t + t.transpose(0, 2, 1, 3))      # symmetric in the field indices
-        v = t[:, i, j, :]
+        v = t[self.crystal.species, i, j, :]
         return np.einsum("a,a,ab->b", 1 / np.sqrt(self.m), v.ravel(), self.pols)
 
 

@@ -286,6 +286,40 @@ def test_c4_group_of_uneven_systems_equals_members_alone():
         d.close()
 
 
+def test_c4_zone_centre_group_equals_members_alone():
+    """The same three systems with the nine IR and Raman chains of each on the zone-centre
+    operator (translation average inside the captured step) and the three random chains on
+    the full image set: six group members, two steps, equal to each member alone."""
+    import torch
+    from tdscha_amd import batched
+    from tdscha_amd.lanczos_device import DeviceLanczos
+    print("case: C4 group with zone-centre members, BF16x3, autotuned", flush=True)
+    group, members, ensembles, info = batched.build(precision="bf16x3", max_steps=2,
+                                                    log=lambda *a: print(*a, flush=True),
+                                                    zone_centre=True)
+    assert [(i["n_syms"], i["chains"], i["operator"]) for i in info] == [
+        (48, 9, "zone_centre"), (384, 3, "full"), (48, 9, "zone_centre"), (3072, 3, "full"),
+        (48, 9, "zone_centre"), (6000, 3, "full")]
+    group.run(2)
+    torch.cuda.synchronize()
+    print("group: 2 steps done", flush=True)
+    for m, d in zip(members, ensembles):
+        alone = DeviceLanczos(d, m.nc, "bf16x3", max_steps=2)
+        alone.start(m._psi0)
+        alone.run(2)
+        for ch in range(m.nc):
+            for x, y in zip(m.coefficients(ch, truncated=False),
+                            alone.coefficients(ch, truncated=False)):
+                assert len(x) == 2 and np.all(np.isfinite(x))
+                assert np.array_equal(x, y)
+        alone.close()
+    group.close()
+    for m in members:
+        m.close()
+    for d in ensembles:
+        d.close()
+
+
 # ---------------------------------------------------------------------------- C5
 
 def test_c5_gradients_against_central_differences(c2_anh):
