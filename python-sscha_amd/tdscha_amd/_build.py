@@ -13,6 +13,7 @@ CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include")
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libtdscha_hip.so")
+TORCH_OPS_PATH = os.path.join(LIB_DIR, "libtdscha_torch.so")
 
 SOURCES = ["tdscha_hip.cpp", "rotate.hip", "gemm.hip", "gemm_bf16.hip", "gemm_bf16_rp.hip", "operator.hip",
            "lanczos.hip", "project.hip", "symmetry.hip", "perturb.hip", "images.hip",
@@ -59,7 +60,35 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB_PATH)
+    build_torch_ops(cc, verbose)
     return LIB_PATH
+
+
+def build_torch_ops(cc, verbose=False):
+    """The PyTorch-ROCm operators (csrc/torch_ops.cpp) as lib/libtdscha_torch.so, linked
+    against libtdscha_hip.so and torch's own libraries (loaded with torch.ops.load_library)."""
+    import torch
+    tdir = os.path.dirname(torch.__file__)
+    inc = [os.path.join(tdir, "include"), os.path.join(tdir, "include", "torch", "csrc", "api",
+                                                         "include")]
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    src = os.path.join(CSRC, "torch_ops.cpp")
+    tmp = TORCH_OPS_PATH + ".tmp"
+    cmd = [cc, "-O2", "-fPIC", "-shared", "-std=c++17", "-D__HIP_PLATFORM_AMD__=1",
+           "-DUSE_ROCM=1", "-D_GLIBCXX_USE_CXX11_ABI=%d" % abi, "-Wno-unused-result"]
+    for d in inc:
+        cmd += ["-I", d]
+    cmd += [src, "-o", tmp, "-L", LIB_DIR, "-ltdscha_hip", "-L", os.path.join(tdir, "lib"),
+            "-lc10", "-lc10_hip", "-ltorch_cpu", "-ltorch_hip", "-ltorch",
+            "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(tdir, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("torch operator build failed: %s\n%s\n%s" % (" ".join(cmd),
+                                                                          r.stdout, r.stderr))
+    if verbose and r.stderr.strip():
+        print(r.stderr)
+    os.replace(tmp, TORCH_OPS_PATH)
+    return TORCH_OPS_PATH
 
 
 if __name__ == "__main__":

@@ -1,12 +1,16 @@
-"""ctypes binding of libtdscha_hip.so (the C ABI declared in include/tdscha_hip.h).
+"""Bindings of libtdscha_hip.so (the C ABI declared in include/tdscha_hip.h).
 
-The library is loaded from the package's lib/ directory. There is no fallback: if the
-shared object is missing, every entry point raises.
+Two layers, both loaded from the package's lib/ directory, with no fallback (if a shared
+object is missing, the entry points that need it raise):
+  * ctypes on the C ABI for set-up, teardown and the host-pointer drop-ins (lib());
+  * the PyTorch-ROCm operators torch.ops.tdscha.* (libtdscha_torch.so, csrc/torch_ops.cpp)
+    for the per-step device calls: they take device tensors, check them, and enqueue the
+    kernels on the caller's current HIP stream (ops()).
 """
 import ctypes
 import os
 
-from ._build import LIB_PATH
+from ._build import LIB_PATH, TORCH_OPS_PATH
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
@@ -105,6 +109,22 @@ def profile_query(group):
     return ms.value, cnt.value
 
 _lib = None
+_ops = None
+
+
+def ops():
+    """torch.ops.tdscha, after loading libtdscha_torch.so (raises if it has not been built)."""
+    global _ops
+    if _ops is None:
+        if not os.path.exists(TORCH_OPS_PATH):
+            raise ImportError(
+                "libtdscha_torch.so is not built (%s); run __graft_entry__.build() or "
+                "python -m tdscha_amd._build" % TORCH_OPS_PATH)
+        import torch
+        lib()                      # the C library first (the operators link against it)
+        torch.ops.load_library(TORCH_OPS_PATH)
+        _ops = torch.ops.tdscha
+    return _ops
 
 
 class TdschaError(RuntimeError):

@@ -1,7 +1,9 @@
 """Device-resident ensemble: the Python face of tdscha_ensemble_* (include/tdscha_hip.h).
 
 PyTorch supplies device memory and the current HIP stream; all arithmetic happens in the
-HIP kernels of libtdscha_hip.so.
+HIP kernels of libtdscha_hip.so. The per-step calls (the perturbed averages, L, the force
+adjoint) go through the PyTorch-ROCm operators torch.ops.tdscha.*; set-up and teardown
+through the C ABI with ctypes.
 """
 import ctypes
 
@@ -150,11 +152,8 @@ class DeviceEnsemble:
             d2v = t.zeros((n, n), dtype=t.float64, device=self.device)
         else:
             self._check_out(d2v, (n, n), "d2v")
-        ptr = lambda x: ctypes.c_void_p(x.data_ptr()) if x is not None else None
-        rc = L.lib().tdscha_ensemble_apply(self._h, L.PRECISIONS[precision], int(terms), ptr(R1),
-                                           ptr(Y1), ptr(f), ptr(d2v), int(bool(d2v_accumulate)),
-                                           _stream_handle(t, self.device))
-        L.check(rc, "tdscha_ensemble_apply")
+        L.ops().ensemble_apply(self._h.value, L.PRECISIONS[precision], int(terms), R1, Y1, f, d2v,
+                               bool(d2v_accumulate))
         return f, d2v
 
     def apply_full_L(self, psi, flags=0, shift=0.0, precision="fp64", out=None):
@@ -168,11 +167,8 @@ class DeviceEnsemble:
             self._check_out(out, psi.shape)
             if out.data_ptr() == psi.data_ptr():
                 raise ValueError("out must not alias psi")
-        rc = L.lib().tdscha_apply_full_L(self._h, L.PRECISIONS[precision], int(flags), float(shift),
-                                         ctypes.c_void_p(psi.data_ptr()),
-                                         ctypes.c_void_p(out.data_ptr()),
-                                         _stream_handle(t, self.device))
-        L.check(rc, "tdscha_apply_full_L")
+        L.ops().apply_full_L_batched(self._h.value, L.PRECISIONS[precision], [psi], [int(flags)],
+                                     [float(shift)], [out])
         return out
 
     def apply_full_L_batched(self, psis, flags=None, shifts=None, precision="fp64", outs=None):
@@ -194,13 +190,9 @@ class DeviceEnsemble:
                     raise ValueError("an output must not alias an input")
         flags = [0] * C if flags is None else list(flags)
         shifts = [0.0] * C if shifts is None else list(shifts)
-        fl = (ctypes.c_int * C)(*[int(f) for f in flags])
-        sh = (ctypes.c_double * C)(*[float(s) for s in shifts])
-        pp = (ctypes.c_void_p * C)(*[p.data_ptr() for p in psis])
-        po = (ctypes.c_void_p * C)(*[o.data_ptr() for o in outs])
-        rc = L.lib().tdscha_apply_full_L_batched(self._h, L.PRECISIONS[precision], C, fl, sh, pp, po,
-                                                 _stream_handle(t, self.device))
-        L.check(rc, "tdscha_apply_full_L_batched")
+        L.ops().apply_full_L_batched(self._h.value, L.PRECISIONS[precision], list(psis),
+                                     [int(f) for f in flags], [float(s) for s in shifts],
+                                     list(outs))
         return outs
 
     def autotune(self, precision="bf16x3", n_chains=2):
@@ -245,14 +237,8 @@ class DeviceEnsemble:
             accumulate = False
         else:
             self._check_out(out, (self.N, n))
-        fl = (ctypes.c_int * C)(*[int(f) for f in flags])
-        pu = (ctypes.c_void_p * C)(*[u.data_ptr() for u in us])
-        pv = (ctypes.c_void_p * C)(*[v.data_ptr() for v in vs])
         if precision not in ("fp64", "fp32"):
             raise ValueError("the force adjoint runs in fp64 or fp32, not %r" % (precision,))
-        rc = L.lib().tdscha_ensemble_force_grad(self._h, L.PRECISIONS[precision], C, fl, pu, pv,
-                                                out.data_ptr(),
-                                                int(bool(accumulate)),
-                                                _stream_handle(t, self.device))
-        L.check(rc, "tdscha_ensemble_force_grad")
+        L.ops().force_grad(self._h.value, L.PRECISIONS[precision], us, vs,
+                           [int(f) for f in flags], out, bool(accumulate))
         return out

@@ -158,11 +158,7 @@ class TranslationProjector:
             elif (out.dtype != torch.float64 or out.device != psi.device or
                   not out.is_contiguous() or out.shape != psi.shape):
                 raise ValueError("out must be a contiguous float64 tensor like psi")
-            st = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
-            L.check(L.lib().tdscha_projector_apply(self._h, int(bool(adjoint)),
-                                                   ctypes.c_void_p(psi.data_ptr()),
-                                                   ctypes.c_void_p(out.data_ptr()), st),
-                    "tdscha_projector_apply")
+            L.ops().projector_apply(self._h.value, self.n, bool(adjoint), psi, out)
             return out
         if out is None:
             out = torch.empty_like(psi)

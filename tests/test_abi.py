@@ -83,3 +83,15 @@ def test_shim_rejects_bad_apply_d4_before_device_work(mode10):
                              np.zeros((45, 45)), mode10.T, 1, 384, mode10.sym.symmetries,
                              mode10.sym.N_degeneracy, mode10.sym.degenerate_space,
                              mode10.sym.block_id, f, d)
+
+
+def test_torch_operators_load_and_declare_their_schemas():
+    """libtdscha_torch.so (the PyTorch-ROCm operators over the C ABI) loads without a GPU and
+    registers the per-step operators (no compute call here)."""
+    from tdscha_amd import _lib as L
+    ops = L.ops()
+    schemas = {name: str(getattr(ops, name).default._schema) for name in
+               ("apply_full_L_batched", "ensemble_apply", "force_grad", "projector_apply")}
+    assert "Tensor[] psis" in schemas["apply_full_L_batched"]
+    assert "Tensor grad" in schemas["force_grad"]
+    assert "bool adjoint" in schemas["projector_apply"]
