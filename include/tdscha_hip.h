@@ -120,6 +120,59 @@ int tdscha_get_perturb_average_sym(const double *X, const double *Y, const doubl
                                    int N_configs, int N_modes, double *force_out,
                                    double *d2v_out);
 
+/* The host-pointer entry points above and below (not the _noalloc one) keep the device
+ * ensemble of their last call, with a host copy of the inputs it was built from, and reuse
+ * it when the next call's inputs are equal byte for byte (the reference passes the same
+ * ensemble at every Lanczos step). No counterpart in the reference, which reads the host
+ * arrays on every call. enable = 0 releases the kept ensemble (device memory and host copy)
+ * and turns the reuse off, 1 turns it on (the default); returns the previous setting. */
+int tdscha_drop_in_cache(int enable);
+
+/* ---- Range-sliced entries of the Julia level (Modules/tdscha_core.jl, "JL" below) -----
+ * Replace get_perturb_averages_sym (JL:202-229), get_perturb_d2v_averages_sym (JL:232-259)
+ * and get_perturb_f_averages_sym (JL:262-284), with the arguments in their order and the
+ * array sizes made explicit:
+ *   X, Y        the (n_modes, n_configs) column-major matrices the reference passes as
+ *               self.X.T, self.Y.T, i.e. the (n_configs, n_modes) row-major X and Y above;
+ *   symmetries  the zero-padded (n_blocks, n_syms, d_max, d_max) row-major array sym_julia
+ *               (Modules/DynamicalLanczos.py:753-768), block b in its top-left
+ *               n_degeneracies[b] x n_degeneracies[b] corner;
+ *   degenerate_space  (n_blocks, d_max) row-major, deg_julia (0-based mode ids);
+ *   blocks      the mode -> block map (sym_block_id);
+ *   start_index, end_index  1-based inclusive range of bigindex = i n_syms + s + 1 over
+ *               configurations i and symmetries s (JL:71-73).
+ * The images in the range are summed and divided by n_syms sum(rho) over ALL configurations,
+ * so the results of disjoint ranges add up to the full average (the split of
+ * Modules/DynamicalLanczos.py:3083-3103). Outputs are overwritten; d2v_dr2 holds the D3 part
+ * plus, with apply_v4, the D4 part. As in the reference, the force entries (and
+ * get_perturb_averages_sym) reject start_index > end_index (JL:171); the d2v entry then
+ * returns zeros. A range outside [1, n_configs n_syms] is rejected (code 6). FP64 on the
+ * current device, synchronous. */
+int tdscha_jl_get_perturb_averages_sym(const double *X, const double *Y, const double *omega,
+                                       const double *rho, const double *R1, const double *Y1,
+                                       double temperature, int apply_v4, const double *symmetries,
+                                       const int *n_degeneracies, const int *degenerate_space,
+                                       const int *blocks, long long start_index,
+                                       long long end_index, int n_modes, int n_configs,
+                                       int n_syms, int n_blocks, int d_max, double *f_average,
+                                       double *d2v_dr2);
+int tdscha_jl_get_perturb_d2v_averages_sym(const double *X, const double *Y, const double *omega,
+                                           const double *rho, const double *R1, const double *Y1,
+                                           double temperature, int apply_v4,
+                                           const double *symmetries, const int *n_degeneracies,
+                                           const int *degenerate_space, const int *blocks,
+                                           long long start_index, long long end_index,
+                                           int n_modes, int n_configs, int n_syms, int n_blocks,
+                                           int d_max, double *d2v_dr2);
+int tdscha_jl_get_perturb_f_averages_sym(const double *X, const double *Y, const double *omega,
+                                         const double *rho, const double *R1, const double *Y1,
+                                         double temperature, int apply_v4,
+                                         const double *symmetries, const int *n_degeneracies,
+                                         const int *degenerate_space, const int *blocks,
+                                         long long start_index, long long end_index, int n_modes,
+                                         int n_configs, int n_syms, int n_blocks, int d_max,
+                                         double *f_average);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
  * arguments, contract and FP64 result as tdscha_get_perturb_average_sym, computed with no

@@ -32,6 +32,11 @@ Contents
   latter with CellConstructor (absent here): its exact form (the acoustic sum rule in
   particular) is parity unpinned; the restatement is pinned on invariance under every
   operation, idempotence, the sum rule and group-invariant model tensors.
+* The Julia level's range-sliced averages over the flattened (configuration, symmetry) index
+  (``perturb_averages_range``, Modules/tdscha_core.jl:54-284) and the reference's split of
+  that index over ranks (``julia_split``, Py:3083-3099). Pinned: the full range equals
+  ``perturb_average_sym``; the split's pieces add up to it; a single image equals the
+  reference's own C build on that one configuration and symmetry.
 * The translation average of a psi vector (``translation_average_psi``): the projector on
   translation-invariant perturbations, restated with explicit loops over the translations'
   atom permutations; the checker of the zone-centre reduction of the image sum.
@@ -264,6 +269,75 @@ def ref_perturb_average_sym(X, Y, w, rho, R1, Y1, T, apply_d4, sym: SymBlocks,
             os.dup2(saved, 1)
             os.close(saved)
     return f, d2v
+
+
+def image_range_pieces(start, end, n_syms):
+    """The images bigindex in [start, end] (1-based, inclusive) as at most three rectangles
+    ((c0, c1), (s0, s1)) of configurations x symmetries: bigindex = i n_syms + s + 1, i.e.
+    configuration (bigindex - 1) // n_syms and symmetry (bigindex - 1) % n_syms
+    (Modules/tdscha_core.jl:71-73, :124-126, :175-177)."""
+    lo, hi = start - 1, end - 1
+    i0, j0 = divmod(lo, n_syms)
+    i1, j1 = divmod(hi, n_syms)
+    if i0 == i1:
+        return [((i0, i0 + 1), (j0, j1 + 1))]
+    pieces = [((i0, i0 + 1), (j0, n_syms))]
+    if i1 > i0 + 1:
+        pieces.append(((i0 + 1, i1), (0, n_syms)))
+    pieces.append(((i1, i1 + 1), (0, j1 + 1)))
+    return pieces
+
+
+def perturb_averages_range(X, Y, w, rho, R1, Y1, T, apply_v4, sym: SymBlocks, start, end,
+                           what="both"):
+    """The Julia level's range-sliced averages (Modules/tdscha_core.jl, "JL"):
+    what = "both"  get_perturb_averages_sym      (JL:202-229) -> (f, d2v)
+           "d2v"   get_perturb_d2v_averages_sym  (JL:232-259) -> d2v
+           "f"     get_perturb_f_averages_sym    (JL:262-284) -> f
+    over the images with bigindex in [start, end], each term as JL:54-198 (the same per-image
+    arithmetic as the C trio), divided by n_syms sum(rho) over every configuration (JL:98,
+    JL:150, JL:197). The force asserts 1 <= start <= end <= N n_syms (JL:171-173); the d2v
+    loops are simply empty when start > end."""
+    N = np.asarray(X).shape[0]
+    n = np.asarray(X).shape[1]
+    nsym = sym.n_syms
+    want_f = what in ("both", "f")
+    want_d = what in ("both", "d2v")
+    if want_f and not (1 <= start <= end <= N * nsym):
+        raise AssertionError("image range [%d, %d] outside [1, %d]" % (start, end, N * nsym))
+    if start > end:
+        return np.zeros((n, n))
+    if not (1 <= start and end <= N * nsym):
+        raise AssertionError("image range [%d, %d] outside [1, %d]" % (start, end, N * nsym))
+    terms = (F_TERM if want_f else 0) | ((D2V_R_TERM | (D2V_Y_TERM if apply_v4 else 0))
+                                        if want_d else 0)
+    norm = (float(np.sum(rho)), float(nsym))
+    f = np.zeros(n)
+    d2v = np.zeros((n, n))
+    for cr, sr in image_range_pieces(start, end, nsym):
+        fp, dp = perturb_average_sym(X, Y, w, rho, R1, Y1, T, apply_v4, sym, cfg_range=cr,
+                                     sym_range=sr, what=terms, norm=norm)
+        f += fp
+        d2v += dp
+    if what == "both":
+        return f, d2v
+    return d2v if what == "d2v" else f
+
+
+def julia_split(n_total, n_processors):
+    """The [start, end] ranges of every rank (Modules/DynamicalLanczos.py:3083-3099): the
+    first n_total % P ranks take one image more."""
+    count, rem = divmod(n_total, n_processors)
+    out = []
+    for rank in range(n_processors):
+        if rank < rem:
+            start = rank * (count + 1)
+            stop = start + count + 1
+        else:
+            start = rank * count + rem
+            stop = start + count
+        out.append((start + 1, stop))
+    return out
 
 
 # --------------------------------------------------------------------------

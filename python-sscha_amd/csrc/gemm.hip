@@ -54,6 +54,7 @@ struct Gemm1Args {
   const T *Ys;       // rotated Y1 in the arithmetic type [sb x npd x npd]
   const double *X, *Ft, *rho, *G;
   double *W;
+  size_t rho_stride;   // rho of slot sl at rho + sl rho_stride (0: one rho for all slots)
   int Np, npd;
   int do_q;
   int raw;           // write alpha, beta, gamma, delta themselves instead of the weights
@@ -217,8 +218,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm1(Gemm1Args<T> g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) Ws[(size_t)q * g.Np + i] = red[0][m][q] + red[1][m][q];
     } else {
-      write_weights(Ws, g.Np, i, g.rho[i], red[0][m][0] + red[1][m][0],
-                    red[0][m][1] + red[1][m][1], red[0][m][2] + red[1][m][2],
+      write_weights(Ws, g.Np, i, g.rho[(size_t)sl * g.rho_stride + i],
+                    red[0][m][0] + red[1][m][0], red[0][m][1] + red[1][m][1],
+                    red[0][m][2] + red[1][m][2],
                     red[0][m][3] + red[1][m][3], g.kf, g.kr, g.ky);
     }
   }
@@ -517,6 +519,23 @@ __global__ void k_grad_out(const double *__restrict__ gp, double *__restrict__ g
 
 }  // namespace
 
+// rho_s[j][i] = rho[i] when i N_sym + sym(j) lies in [lo, hi], else 0 (the images of a
+// range-sliced call, Modules/tdscha_core.jl:71-73: bigindex k -> config (k - 1) / N_sym,
+// symmetry (k - 1) % N_sym)
+__global__ void k_window_rho(const double *__restrict__ rho, double *__restrict__ rho_s, int Np,
+                             int N_sym, Pass ps, long long lo, long long hi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= Np) return;
+  const long long k = (long long)i * N_sym + ps.sym(j);
+  rho_s[(size_t)j * Np + i] = (k >= lo && k <= hi) ? rho[i] : 0.0;
+}
+
+void launch_window_rho(const Ensemble &e, const Pass &ps, hipStream_t st) {
+  hipLaunchKernelGGL(k_window_rho, dim3((e.Np + 255) / 256, ps.slots()), dim3(256), 0, st, e.rho,
+                     e.rho_s, e.Np, e.N_sym, ps, e.win_lo, e.win_hi);
+}
+
 void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st) {
   const int do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   const double kf = (what & kTermF) ? 1.0 : 0.0;
@@ -527,13 +546,15 @@ void launch_gemm1(const Ensemble &e, int sb, int prec, int what, hipStream_t st)
   } else if (prec == kFP32) {
     Gemm1Args<float> g;
     g.A = e.X_f; g.Ys = e.Ys_f32;
-    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
+    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho_w(); g.rho_stride = e.rho_stride(); g.G = e.G;
+    g.W = e.W;
     g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.raw = 0; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
   } else {
     Gemm1Args<double> g;
     g.A = e.X; g.Ys = e.Ys;
-    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = e.G; g.W = e.W;
+    g.X = e.X; g.Ft = e.Ft; g.rho = e.rho_w(); g.rho_stride = e.rho_stride(); g.G = e.G;
+    g.W = e.W;
     g.Np = e.Np; g.npd = e.npd; g.do_q = do_q; g.raw = 0; g.kf = kf; g.kr = kr; g.ky = ky;
     hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
   }
@@ -543,7 +564,7 @@ void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *
                       hipStream_t st) {
   Gemm1Args<double> g;
   g.A = e.X; g.Ys = M;
-  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = G; g.W = out;
+  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.rho_stride = 0; g.G = G; g.W = out;
   g.Np = e.Np; g.npd = e.npd; g.do_q = 1; g.raw = 1; g.kf = g.kr = g.ky = 1.0;
   hipLaunchKernelGGL(k_gemm1<double>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
 }
@@ -552,7 +573,7 @@ void launch_gemm1_raw_f32(const Ensemble &e, int sb, const float *M, const doubl
                           double *out, hipStream_t st) {
   Gemm1Args<float> g;
   g.A = e.X_f; g.Ys = M;
-  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.G = G; g.W = out;
+  g.X = e.X; g.Ft = e.Ft; g.rho = e.rho; g.rho_stride = 0; g.G = G; g.W = out;
   g.Np = e.Np; g.npd = e.npd; g.do_q = 1; g.raw = 1; g.kf = g.kr = g.ky = 1.0;
   hipLaunchKernelGGL(k_gemm1<float>, dim3(e.Np / BM, sb), dim3(256), 0, st, g);
 }
@@ -584,7 +605,8 @@ void launch_fav(const Ensemble &e, int sb, hipStream_t st) {
 void launch_gemm1_images(const Ensemble &e, const double *Ysym_c, int what, hipStream_t st) {
   Gemm1Args<double> g;
   g.A = e.IU; g.Ys = Ysym_c;
-  g.X = e.IU; g.Ft = e.IFt; g.rho = e.rho; g.G = e.G; g.W = e.W;
+  g.X = e.IU; g.Ft = e.IFt; g.rho = e.rho_w(); g.rho_stride = e.rho_stride(); g.G = e.G;
+  g.W = e.W;
   g.Np = e.Np; g.npd = e.npd;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
   g.raw = 0;

@@ -217,10 +217,11 @@ __global__ __launch_bounds__(2 * TN, 256 / TN) void k_gemm1_bf16x3(Gemm1Args g) 
 // W[s] = (c1, c2, e1, e2) from the column-tile row sums, tiles added in increasing order.
 // Wf (when not null) receives c1 and c2 / 2 in FP32, the weights as the GEMM2 operand
 // formation uses them.
+// rho of slot sl at rho + sl rho_stride (0: one rho for every slot).
 __global__ void k_weights_bf16(const double *__restrict__ P, const double *__restrict__ GD,
-                               const double *__restrict__ rho, double *__restrict__ W,
-                               float *__restrict__ Wf, int Np, int n_ct, int nq, int sb,
-                               int gd_stride, double kf, double kr, double ky) {
+                               const double *__restrict__ rho, size_t rho_stride,
+                               double *__restrict__ W, float *__restrict__ Wf, int Np, int n_ct,
+                               int nq, int sb, int gd_stride, double kf, double kr, double ky) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int sl = blockIdx.y;
   if (i >= Np) return;
@@ -232,7 +233,7 @@ __global__ void k_weights_bf16(const double *__restrict__ P, const double *__res
     s[2] = GD[(size_t)sl * Np + i];
     s[3] = GD[((size_t)gd_stride + sl) * Np + i];
   }
-  const double r = rho[i];
+  const double r = rho[(size_t)sl * rho_stride + i];
   double *Ws = W + (size_t)sl * 4 * Np;
   const double c1 = r * (ky * s[0] / 8 - kr * s[2] / 3);
   const double c2 = r * (ky * s[1] / 4 - kr * s[3] / 3);
@@ -668,8 +669,8 @@ void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, 
   const double kr = (what & kTermD2vR) ? 1.0 : 0.0;
   const double ky = (what & kTermD2vY) ? 1.0 : 0.0;
   hipLaunchKernelGGL(k_weights_bf16, dim3((e.Np + 255) / 256, sb), dim3(256), 0, st, e.Pw,
-                     kr != 0.0 ? e.GD : nullptr, e.rho, e.W, e.Wf, e.Np, n_ct, nq, sb, e.s_batch,
-                     kf, kr, ky);
+                     kr != 0.0 ? e.GD : nullptr, e.rho_w(), e.rho_stride(), e.W, e.Wf, e.Np,
+                     n_ct, nq, sb, e.s_batch, kf, kr, ky);
 }
 
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {

@@ -4,6 +4,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -207,6 +209,7 @@ int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st, in
         Span sp(TDSCHA_KGROUP_ROTATE, st);
         launch_images(e, s, st);
       }
+      if (e.win) launch_window_rho(e, Pass{s, 1, 1}, st);
       for (int c = 0; c < nc; ++c) {
         if (need_r) launch_image_g(e, c, st);
         {
@@ -236,6 +239,7 @@ int run_passes(const Ensemble &e, int prec, int what, int nc, hipStream_t st, in
   for (int s0 = s_begin; s0 < s_end; s0 += ns_max) {
     const Pass ps{s0, (s_end - s0) < ns_max ? (s_end - s0) : ns_max, nc};
     const int sb = ps.slots();
+    if (e.win) launch_window_rho(e, ps, st);
     {
       Span sp(TDSCHA_KGROUP_ROTATE, st);
       if (need_q && e.fused_rot) {
@@ -305,6 +309,7 @@ int alloc_slots(tdscha_ensemble *h, int slots) {
   }
   e.Ys_f32 = nullptr;
   e.Ys_bf = nullptr;
+  e.rho_s = nullptr;   // sized by s_batch: allocated again by the next range-sliced call
   if (e.prec_mask & (1 << kFP32)) rc |= dev_alloc(h, &e.Ys_f32, sb * nn);
   if (e.prec_mask & (1 << kBF16x3)) rc |= dev_alloc(h, &e.Ys_bf, 2 * sb * nn);
   if (rc) return rc;
@@ -895,29 +900,108 @@ int tdscha_ensemble_force_grad(tdscha_ensemble *h, int precision, int n_pairs, c
 
 // ---- host-pointer drop-ins --------------------------------------------------------------
 
-static int host_call(const double *X, const double *Y, const double *w, const double *rho,
-                     const double *R1, const double *Y1, double T, int terms, int n, int N,
-                     int N_sym, int n_blocks, const double *const *syms, const int *ndeg,
-                     const int *const *degs, const int *bids, double *f_out, double *d2v_out,
-                     int d2v_accumulate) {
-  if (!X || !Y || !w || !rho || (!f_out && (terms & kTermF)) ||
-      (!d2v_out && (terms & (kTermD2vR | kTermD2vY)))) {
-    set_error("NULL argument");
-    return 1;
-  }
-  int dev = 0;
-  TD_CHECK(hipGetDevice(&dev));
+}  // extern "C"
+
+namespace {
+
+// The host-pointer entry points receive the whole ensemble on every call: the reference
+// recomputes from host arrays at every Lanczos step. The ensemble built by the last such call
+// stays on the device together with a host copy of every input it was built from, and a
+// call whose inputs are equal to those byte for byte (and whose sizes, temperature and
+// configuration window are the same) reuses it instead of uploading and converting again.
+// Comparing costs one pass over the inputs on the host; the copy holds as much host memory
+// as the inputs. tdscha_drop_in_cache(0) releases both and turns the cache off.
+struct Blob {
+  const void *p;
+  size_t bytes;
+};
+
+struct DropInCache {
+  bool enabled = true;
   tdscha_ensemble *h = nullptr;
-  int rc = tdscha_ensemble_create(&h, dev, X, Y, w, rho, T, n, N, N_sym, n_blocks, syms, ndeg, degs,
-                                  bids, 1, 0);
-  if (rc) return rc;
+  std::vector<long long> key;
+  std::vector<std::vector<char>> blobs;
+
+  bool matches(const std::vector<long long> &k, const std::vector<Blob> &in) const {
+    if (!h || k != key || in.size() != blobs.size()) return false;
+    for (size_t b = 0; b < in.size(); ++b)
+      if (in[b].bytes != blobs[b].size() ||
+          (in[b].bytes && std::memcmp(in[b].p, blobs[b].data(), in[b].bytes) != 0))
+        return false;
+    return true;
+  }
+  void clear() {
+    if (h) tdscha_ensemble_destroy(h);
+    h = nullptr;
+    key.clear();
+    std::vector<std::vector<char>>().swap(blobs);
+  }
+  // keep nh (and a copy of its inputs); false when the host copy cannot be made
+  bool store(tdscha_ensemble *nh, const std::vector<long long> &k, const std::vector<Blob> &in) {
+    clear();
+    try {
+      blobs.resize(in.size());
+      for (size_t b = 0; b < in.size(); ++b) {
+        const char *p = static_cast<const char *>(in[b].p);
+        blobs[b].assign(p, p + in[b].bytes);
+      }
+    } catch (const std::bad_alloc &) {
+      std::vector<std::vector<char>>().swap(blobs);
+      return false;
+    }
+    h = nh;
+    key = k;
+    return true;
+  }
+};
+
+std::mutex g_cache_mu;
+// never destroyed: its device buffers are not freed after the HIP runtime has shut down
+DropInCache *g_cache = new DropInCache;
+
+long long bits_of(double v) {
+  long long b;
+  std::memcpy(&b, &v, 8);
+  return b;
+}
+
+// Evaluate `terms` on the ensemble that make() builds, or on the cached one built from the
+// same inputs (held under g_cache_mu by the caller). With win, only the images whose
+// flattened index lies in [win_lo, win_hi] count.
+int drop_in_apply(const std::vector<long long> &key, const std::vector<Blob> &inputs,
+                  const std::function<int(tdscha_ensemble **)> &make, bool win, long long win_lo,
+                  long long win_hi, int n, const double *R1, const double *Y1, int terms,
+                  double *f_out, double *d2v_out, int d2v_accumulate) {
+  tdscha_ensemble *h = nullptr;
+  bool owned = true;
+  int rc = 0;
+  if (g_cache->enabled && g_cache->matches(key, inputs)) {
+    h = g_cache->h;
+    owned = false;
+  } else {
+    if (g_cache->h) g_cache->clear();   // its device memory goes before the next ensemble's
+    rc = make(&h);
+    if (rc) return rc;
+    if (g_cache->enabled && g_cache->store(h, key, inputs)) owned = false;
+  }
+  Ensemble &e = h->e;
+  if (hipSetDevice(e.device) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    rc = 10;
+  }
+  if (!rc && win && !e.rho_s) rc = dev_alloc(h, &e.rho_s, (size_t)e.s_batch * e.Np);
+  if (rc) {
+    if (owned) tdscha_ensemble_destroy(h); else g_cache->clear();
+    return rc;
+  }
   double *dR1 = nullptr, *dY1 = nullptr, *df = nullptr, *dd = nullptr;
   auto cleanup = [&]() {
     if (dR1) (void)hipFree(dR1);
     if (dY1) (void)hipFree(dY1);
     if (df) (void)hipFree(df);
     if (dd) (void)hipFree(dd);
-    tdscha_ensemble_destroy(h);
+    e.win = false;
+    if (owned) tdscha_ensemble_destroy(h);
   };
   const size_t nn = (size_t)n * n;
   if (hipMalloc(&dR1, n * 8) != hipSuccess || hipMalloc(&dY1, nn * 8) != hipSuccess ||
@@ -936,6 +1020,9 @@ static int host_call(const double *X, const double *Y, const double *w, const do
     cleanup();
     return 10;
   }
+  e.win = win;
+  e.win_lo = win_lo;
+  e.win_hi = win_hi;
   rc = tdscha_ensemble_apply(h, kFP64, terms, dR1, dY1, df, dd, d2v_accumulate, nullptr);
   if (!rc) {
     err = hipDeviceSynchronize();
@@ -947,8 +1034,189 @@ static int host_call(const double *X, const double *Y, const double *w, const do
       rc = 10;
     }
   }
+  if (rc && !owned) g_cache->clear();   // an ensemble whose call failed is not reused
   cleanup();
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+static int host_call(const double *X, const double *Y, const double *w, const double *rho,
+                     const double *R1, const double *Y1, double T, int terms, int n, int N,
+                     int N_sym, int n_blocks, const double *const *syms, const int *ndeg,
+                     const int *const *degs, const int *bids, double *f_out, double *d2v_out,
+                     int d2v_accumulate) {
+  if (!X || !Y || !w || !rho || !syms || !ndeg || !degs || !bids ||
+      (!f_out && (terms & kTermF)) || (!d2v_out && (terms & (kTermD2vR | kTermD2vY)))) {
+    set_error("NULL argument");
+    return 1;
+  }
+  if (n <= 0 || N <= 0 || N_sym <= 0 || n_blocks <= 0) {
+    set_error("n_modes, n_configs, N_sym and the number of blocks must be positive");
+    return 1;
+  }
+  for (int b = 0; b < n_blocks; ++b)
+    if (!syms[b] || !degs[b] || ndeg[b] <= 0 || ndeg[b] > n) {
+      set_error("symmetry block " + std::to_string(b) + " is NULL or its size is not in [1, n_modes]");
+      return 2;
+    }
+  int dev = 0;
+  TD_CHECK(hipGetDevice(&dev));
+  const size_t nd = (size_t)n * N * 8;
+  std::vector<Blob> in = {{X, nd}, {Y, nd}, {w, (size_t)n * 8}, {rho, (size_t)N * 8},
+                          {ndeg, (size_t)n_blocks * 4}, {bids, (size_t)n * 4}};
+  for (int b = 0; b < n_blocks; ++b) {
+    in.push_back({syms[b], (size_t)N_sym * ndeg[b] * ndeg[b] * 8});
+    in.push_back({degs[b], (size_t)ndeg[b] * 4});
+  }
+  const std::vector<long long> key = {1, dev, n, N, N_sym, n_blocks, bits_of(T)};
+  auto make = [&](tdscha_ensemble **h) {
+    return tdscha_ensemble_create(h, dev, X, Y, w, rho, T, n, N, N_sym, n_blocks, syms, ndeg, degs,
+                                  bids, 1, 0);
+  };
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  return drop_in_apply(key, in, make, false, 0, -1, n, R1, Y1, terms, f_out, d2v_out,
+                       d2v_accumulate);
+}
+
+int tdscha_drop_in_cache(int enable) {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  const int was = g_cache->enabled ? 1 : 0;
+  g_cache->clear();
+  g_cache->enabled = enable != 0;
+  return was;
+}
+
+// ---- range-sliced entries of the Julia level (Modules/tdscha_core.jl, "JL") -------------
+// The images are numbered bigindex = i N_sym + s + 1 (configuration i, symmetry s,
+// JL:71-73); a call sums those in [start_index, end_index] (1-based, inclusive) and divides
+// by N_sym sum(rho) over every configuration, so that the results of disjoint ranges add up
+// to the full average (the split of Py:3083-3103). The device evaluates the configurations
+// that the range touches (i from (start - 1) / N_sym to (end - 1) / N_sym), with the images
+// outside the range given zero weight.
+static int range_call(const double *X, const double *Y, const double *w, const double *rho,
+                      const double *R1, const double *Y1, double T, int terms,
+                      const double *sym_pad, const int *ndeg, const int *deg_pad, const int *bids,
+                      long long start, long long end, int n, int N, int N_sym, int n_blocks,
+                      int d_max, double *f_out, double *d2v_out) {
+  if (!X || !Y || !w || !rho || !sym_pad || !ndeg || !deg_pad || !bids ||
+      (!f_out && (terms & kTermF)) || (!d2v_out && (terms & (kTermD2vR | kTermD2vY)))) {
+    set_error("NULL argument");
+    return 1;
+  }
+  if (n <= 0 || N <= 0 || N_sym <= 0 || n_blocks <= 0 || d_max <= 0) {
+    set_error("n_modes, n_configs, n_syms, n_blocks and d_max must be positive");
+    return 1;
+  }
+  for (int b = 0; b < n_blocks; ++b)
+    if (ndeg[b] <= 0 || ndeg[b] > d_max) {
+      set_error("n_degeneracies[" + std::to_string(b) + "] is not in [1, d_max]");
+      return 2;
+    }
+  const long long total = (long long)N * N_sym;
+  if (start > end) {
+    // JL:171-173 assert start <= end for the force; the d2v loops then run zero times
+    if (terms & kTermF) {
+      set_error("Error, start_index " + std::to_string(start) + " should be lower than end " +
+                std::to_string(end));
+      return 6;
+    }
+    std::memset(d2v_out, 0, (size_t)n * n * 8);
+    return 0;
+  }
+  if (start < 1 || end > total) {
+    set_error("the image range [" + std::to_string(start) + ", " + std::to_string(end) +
+              "] is not inside [1, n_configs * n_syms = " + std::to_string(total) + "]");
+    return 6;
+  }
+  int dev = 0;
+  TD_CHECK(hipGetDevice(&dev));
+  const long long lo = start - 1, hi = end - 1;
+  const int i0 = (int)(lo / N_sym), i1 = (int)(hi / N_sym);
+  const int Nsub = i1 - i0 + 1;
+  double neff = 0.0;
+  for (int i = 0; i < N; ++i) neff += rho[i];
+  const double *Xs = X + (size_t)i0 * n, *Ys = Y + (size_t)i0 * n, *rs = rho + i0;
+  const size_t nd = (size_t)n * Nsub * 8;
+  const std::vector<Blob> in = {{Xs, nd}, {Ys, nd}, {w, (size_t)n * 8}, {rs, (size_t)Nsub * 8},
+                                {sym_pad, (size_t)n_blocks * N_sym * d_max * d_max * 8},
+                                {ndeg, (size_t)n_blocks * 4}, {deg_pad, (size_t)n_blocks * d_max * 4},
+                                {bids, (size_t)n * 4}};
+  const std::vector<long long> key = {2, dev, n, N, N_sym, n_blocks, d_max, i0, i1, bits_of(T),
+                                      bits_of(neff)};
+  auto make = [&](tdscha_ensemble **h) {
+    // the zero-padded (n_blocks, N_sym, d_max, d_max) blocks (Py:753-768) as the
+    // (N_sym, d_b, d_b) block list the ensemble takes
+    std::vector<std::vector<double>> blocks(n_blocks);
+    std::vector<const double *> sp(n_blocks);
+    std::vector<const int *> dp(n_blocks);
+    try {
+      for (int b = 0; b < n_blocks; ++b) {
+        const int d = ndeg[b];
+        blocks[b].resize((size_t)N_sym * d * d);
+        for (int s = 0; s < N_sym; ++s)
+          for (int a = 0; a < d; ++a)
+            std::memcpy(&blocks[b][((size_t)s * d + a) * d],
+                        sym_pad + (((size_t)b * N_sym + s) * d_max + a) * d_max, (size_t)d * 8);
+        sp[b] = blocks[b].data();
+        dp[b] = deg_pad + (size_t)b * d_max;
+      }
+    } catch (const std::bad_alloc &) {
+      set_error("out of host memory for the symmetry blocks");
+      return 11;
+    }
+    int rc = tdscha_ensemble_create(h, dev, Xs, Ys, w, rs, T, n, Nsub, N_sym, n_blocks, sp.data(),
+                                    ndeg, dp.data(), bids, 1, 0);
+    if (!rc) (*h)->e.N_eff = neff;   // normalised over every configuration, not the slice
+    return rc;
+  };
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  return drop_in_apply(key, in, make, true, lo - (long long)i0 * N_sym,
+                       hi - (long long)i0 * N_sym, n, R1, Y1, terms, f_out, d2v_out, 0);
+}
+
+int tdscha_jl_get_perturb_averages_sym(const double *X, const double *Y, const double *omega,
+                                       const double *rho, const double *R1, const double *Y1,
+                                       double temperature, int apply_v4, const double *symmetries,
+                                       const int *n_degeneracies, const int *degenerate_space,
+                                       const int *blocks, long long start_index,
+                                       long long end_index, int n_modes, int n_configs,
+                                       int n_syms, int n_blocks, int d_max, double *f_average,
+                                       double *d2v_dr2) {
+  const int terms = kTermF | kTermD2vR | (apply_v4 ? kTermD2vY : 0);
+  return range_call(X, Y, omega, rho, R1, Y1, temperature, terms, symmetries, n_degeneracies,
+                    degenerate_space, blocks, start_index, end_index, n_modes, n_configs, n_syms,
+                    n_blocks, d_max, f_average, d2v_dr2);
+}
+
+int tdscha_jl_get_perturb_d2v_averages_sym(const double *X, const double *Y, const double *omega,
+                                           const double *rho, const double *R1, const double *Y1,
+                                           double temperature, int apply_v4,
+                                           const double *symmetries, const int *n_degeneracies,
+                                           const int *degenerate_space, const int *blocks,
+                                           long long start_index, long long end_index,
+                                           int n_modes, int n_configs, int n_syms, int n_blocks,
+                                           int d_max, double *d2v_dr2) {
+  const int terms = kTermD2vR | (apply_v4 ? kTermD2vY : 0);
+  return range_call(X, Y, omega, rho, R1, Y1, temperature, terms, symmetries, n_degeneracies,
+                    degenerate_space, blocks, start_index, end_index, n_modes, n_configs, n_syms,
+                    n_blocks, d_max, nullptr, d2v_dr2);
+}
+
+int tdscha_jl_get_perturb_f_averages_sym(const double *X, const double *Y, const double *omega,
+                                         const double *rho, const double *R1, const double *Y1,
+                                         double temperature, int apply_v4,
+                                         const double *symmetries, const int *n_degeneracies,
+                                         const int *degenerate_space, const int *blocks,
+                                         long long start_index, long long end_index, int n_modes,
+                                         int n_configs, int n_syms, int n_blocks, int d_max,
+                                         double *f_average) {
+  (void)apply_v4;   // the force does not depend on it (JL:262-284)
+  return range_call(X, Y, omega, rho, R1, Y1, temperature, kTermF, symmetries, n_degeneracies,
+                    degenerate_space, blocks, start_index, end_index, n_modes, n_configs, n_syms,
+                    n_blocks, d_max, f_average, nullptr);
 }
 
 int tdscha_get_f_average_from_Y_pert_sym_fast(const double *X, const double *Y, const double *w,

@@ -138,6 +138,14 @@ struct Ensemble {
   double *IU = nullptr, *IFt = nullptr;   // [Np x npd]  u = S x, f~ = f_psi S y
   double *IUT = nullptr, *IFT = nullptr;  // [npd x Np]  u~^T = (f_ups S x)^T, f^T = (S y)^T
   double *gradp = nullptr;    // [Np x npd]            dJ/dY, padded
+  // range-sliced evaluation (tdscha_jl_*): only the images whose flattened index
+  // i N_sym + s (configuration i, symmetry s) lies in [win_lo, win_hi] count; the weights
+  // then read rho_s[slot][i], rho[i] inside the window and 0 outside (filled per pass)
+  bool win = false;
+  long long win_lo = 0, win_hi = -1;
+  double *rho_s = nullptr;    // [s_batch x Np]
+  const double *rho_w() const { return win ? rho_s : rho; }
+  size_t rho_stride() const { return win ? (size_t)Np : 0; }
 };
 
 void set_error(const std::string &msg);
@@ -201,6 +209,8 @@ void launch_post_adj(const Ensemble &e, const double *u, double *U, double *uR, 
                      hipStream_t st);
 
 // ---- GEMMs (gemm.hip) ----
+// rho_s of the pass's slots for the image window (e.win)
+void launch_window_rho(const Ensemble &e, const Pass &ps, hipStream_t st);
 // GEMM1 with the rotated matrices M and vectors G of the slots, writing the four row sums
 // themselves (alpha, beta, gamma, delta) to out [sb x 4 x Np]
 void launch_gemm1_raw(const Ensemble &e, int sb, const double *M, const double *G, double *out,

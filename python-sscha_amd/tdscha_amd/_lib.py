@@ -25,6 +25,12 @@ _ipp = ctypes.POINTER(_ip)
 _vp = ctypes.c_void_p
 
 # name -> (restype, argtypes); every symbol declared in include/tdscha_hip.h
+# the Julia-level range entries: X, Y, omega, rho, R1, Y1, temperature, apply_v4, symmetries,
+# n_degeneracies, degenerate_space, blocks, start_index, end_index, n_modes, n_configs,
+# n_syms, n_blocks, d_max (then the outputs)
+_JL = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _ip, _ip,
+       ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+       ctypes.c_int, ctypes.c_int]
 _TRIO = [_dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dpp, ctypes.c_int,
          _ip, _ipp, _ip, _dp]
 SIGNATURES = {
@@ -43,6 +49,10 @@ SIGNATURES = {
         _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "tdscha_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "tdscha_drop_in_cache": (ctypes.c_int, [ctypes.c_int]),
+    "tdscha_jl_get_perturb_averages_sym": (ctypes.c_int, _JL + [_dp, _dp]),
+    "tdscha_jl_get_perturb_d2v_averages_sym": (ctypes.c_int, _JL + [_dp]),
+    "tdscha_jl_get_perturb_f_averages_sym": (ctypes.c_int, _JL + [_dp]),
     "tdscha_ensemble_create": (ctypes.c_int, [
         ctypes.POINTER(_vp), ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int]),
@@ -139,6 +149,11 @@ def lib():
             raise ImportError(
                 "libtdscha_hip.so is not built (%s); run __graft_entry__.build() or "
                 "python -m tdscha_amd._build" % LIB_PATH)
+        # PyTorch first: its wheel ships the HIP runtime under the same soname
+        # (libamdhip64.so.7) as the one the library links, and the process keeps the first
+        # copy it loads. Loaded before torch, the system runtime would serve torch too, and
+        # torch then sees no device.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
