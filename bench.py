@@ -52,9 +52,10 @@ BF16_KERNELS = {
     0: ("k_gemm1_bf16x3<256, 128>", "k_gemm2_bf16x3<192, 128>"),
     1: ("k_gemm1_bf16x3<256, 256>", "k_gemm2_bf16x3<256, 256>"),
     2: ("k_gemm1_bf16x3_dma", "k_gemm2_bf16x3_dma"),
-    3: ("k_gemm1_bf16x3_rp<2>", "k_gemm2_bf16x3_rp<2>"),
-    4: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<1>"),
-    5: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<2>"),
+    3: ("k_gemm1_bf16x3_rp<2>", "k_gemm2_bf16x3_rp<2, 3>"),
+    4: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<1, 3>"),
+    5: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<2, 3>"),
+    6: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<1, 4>"),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bf16x3_rocksalt6.md")
 

@@ -687,7 +687,9 @@ void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
   // forms 3-5 write B_s in FP32 for the fused back-rotation; the one-sided rotations (blocks
   // wider than the group tile) read the FP64 B, which form 0 writes
-  if (e.bf_variant >= 3 && e.fused_rot)
+  if (e.bf_variant == 6 && e.fused_rot)
+    launch_gemm2_bf16x3_rp(e, sb, 1, st, 4);                          // 256 x 96
+  else if (e.bf_variant >= 3 && e.fused_rot)
     launch_gemm2_bf16x3_rp(e, sb, e.bf_variant == 4 ? 1 : 2, st);   // 3, 5: 192 columns
   else if (e.bf_variant >= 3)
     gemm2_launch<192, 128>(e, sb, st);

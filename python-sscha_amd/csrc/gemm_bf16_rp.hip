@@ -198,6 +198,17 @@ __device__ inline void rmma(const RStage<TN> &s, const AFrag<MB> &a, int2 base,
   rmma_from<MB, J0, J1, TN>(s, a, first, base, acc);
 }
 
+// Which halves of a wave's column blocks multiply data (wave-uniform): none when the wave's
+// rows all lie in the zero padding past the last configuration (GEMM1) or mode (GEMM2), the
+// first three blocks when the wave's last three columns blocks lie past the last mode. The
+// skipped products are those of zero padding: every stored value stays as it is.
+struct Halves {
+  bool h0, h1;
+};
+__device__ inline Halves live_halves(int mi, int nj) {
+  return Halves{mi > 0 && nj > 0, mi > 0 && nj > RNB / 2};
+}
+
 // ------------------------------------------------------------------------------ GEMM1
 struct Gemm1RArgs {
   const uint16_t *Xh, *Xl;    // [Np x npd] fragment-tiled
@@ -205,6 +216,7 @@ struct Gemm1RArgs {
   const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
   double *P;                  // [sb x n_ct x 2 x Np] row sums per column tile
   int Np, npd, sb, n_it, n_ct, sg, do_q;   // sg: image slots per tile group
+  int N, n;                   // configurations and modes (the rest is zero padding)
 };
 
 // Row sums of block row ti of this wave over its columns below nj: alpha = sum X Q and
@@ -250,8 +262,9 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int wr = w & 3, wc = w >> 2;
   const int cw = c0 + wc * RWC;                 // first column of this wave
-  const int nj = clamp_blocks(g.npd, cw, RNB);
+  const int nj = clamp_blocks(g.n, cw, RNB);
   const int row0 = i0 + wr * 64;
+  const Halves hv = live_halves(clamp_blocks(g.N, row0, MB), nj);
 
   v4f acc[MB][RNB];
 #pragma unroll
@@ -276,16 +289,16 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm1_bf16x3_rp(Gemm
     for (int kt = 0; kt < KT; kt += 2) {
       // stage 0: slice kt; b1: slice kt + 1; a0: slice kt; a1: slice kt + 1
       rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, min((kt + 2) * TK, last), b0);
-      rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
+      if (hv.h0) rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
       rstore_b<WCN>(st[1], b1);
-      rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
       rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
       __syncthreads();
       // stage 1: slice kt + 1; b0: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
       rload_b<WCN>(Yh, Yl, g.npd, g.npd, c0, min((kt + 3) * TK, last), b1);
-      rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
+      if (hv.h0) rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
       rstore_b<WCN>(st[0], b0);
-      rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
       rload_a<MB>(g.Xh, g.Xl, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
       __syncthreads();
     }
@@ -336,6 +349,8 @@ struct Gemm2RArgs {
   const float *Wf;            // [s_batch x 2 x Np]  c1, c2 / 2
   float *B;                   // [s_batch x npd x npd]
   int Np, npd, sb, n_mt, n_ct;
+  int KT;                     // k slices that hold configurations (even; the rest is padding)
+  int n;                      // modes (rows and columns past it are zero padding)
 };
 
 // Sources of the formed B slice: rows (t / 8) + RPP h (h < 3) of the tile,
@@ -378,9 +393,9 @@ __device__ inline void rstore_g(RStage<RCfg<WCN>::TN> &st, const GSrc &s) {
   }
 }
 
-template <int WCN>
+template <int WCN, int MB>
 __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm2RArgs g) {
-  constexpr int MB = 3, TM = 192, TN = RCfg<WCN>::TN;
+  constexpr int TM = 64 * MB, TN = RCfg<WCN>::TN;
   __shared__ RStage<TN> st[2];
   const int total = g.sb * g.n_mt * g.n_ct;
   const int id = xcd_tile(total);
@@ -394,9 +409,10 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
   const int m0 = mt * TM, c0 = ct * TN;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int wr = w & 3, wc = w >> 2;
-  const int row0 = m0 + wr * 48, cw = c0 + wc * RWC;
-  const int mi = clamp_blocks(g.npd, row0, MB);
-  const int nj = clamp_blocks(g.npd, cw, RNB);
+  const int row0 = m0 + wr * 16 * MB, cw = c0 + wc * RWC;
+  const int mi = clamp_blocks(g.n, row0, MB);
+  const int nj = clamp_blocks(g.n, cw, RNB);
+  const Halves hv = live_halves(mi, nj);
   const float *wf = g.Wf + (size_t)sl * 2 * g.Np;
 
   v4f acc[MB][RNB];
@@ -405,9 +421,10 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
 #pragma unroll
     for (int b = 0; b < RNB; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
 
-  const int KT = g.Np / TK, last = (KT - 1) * TK;   // KT is a multiple of 8
+  const int KT = g.KT, last = (KT - 1) * TK;
   const int2 base = rfrag_base();
-  const int n_ks = KT, n_rb = g.npd / 16, rb0 = row0 / 16;
+  // the fragment-tiled layout holds all Np / TK slices; the loop stops after KT of them
+  const int n_ks = g.Np / TK, n_rb = g.npd / 16, rb0 = row0 / 16;
   GSrc src;
   AFrag<MB> a0, a1;
   rload_g<WCN>(g, wf, c0, 0, src);
@@ -416,21 +433,34 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
   rstore_g<WCN>(st[0], src);
   rload_g<WCN>(g, wf, c0, TK, src);
   __syncthreads();
-  for (int kt = 0; kt < KT; kt += 2) {
-    // stage 0: slice kt; src: slice kt + 1; a0: slice kt; a1: slice kt + 1
-    rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
-    rstore_g<WCN>(st[1], src);
-    rload_g<WCN>(g, wf, c0, min((kt + 2) * TK, last), src);
-    rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
-    rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
-    __syncthreads();
-    // stage 1: slice kt + 1; src: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
-    rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
-    rstore_g<WCN>(st[0], src);
-    rload_g<WCN>(g, wf, c0, min((kt + 3) * TK, last), src);
-    rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
-    rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
-    __syncthreads();
+  if (hv.h0) {
+    for (int kt = 0; kt < KT; kt += 2) {
+      // stage 0: slice kt; src: slice kt + 1; a0: slice kt; a1: slice kt + 1
+      rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
+      rstore_g<WCN>(st[1], src);
+      rload_g<WCN>(g, wf, c0, min((kt + 2) * TK, last), src);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
+      rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
+      __syncthreads();
+      // stage 1: slice kt + 1; src: slice kt + 2; a1: slice kt + 1; a0: slice kt + 2
+      rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
+      rstore_g<WCN>(st[0], src);
+      rload_g<WCN>(g, wf, c0, min((kt + 3) * TK, last), src);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
+      rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
+      __syncthreads();
+    }
+  } else {
+    // a wave whose rows or columns all lie past the last mode stages its share of the
+    // formed operand and keeps the barriers, but issues no MFMAs
+    for (int kt = 0; kt < KT; kt += 2) {
+      rstore_g<WCN>(st[1], src);
+      rload_g<WCN>(g, wf, c0, min((kt + 2) * TK, last), src);
+      __syncthreads();
+      rstore_g<WCN>(st[0], src);
+      rload_g<WCN>(g, wf, c0, min((kt + 3) * TK, last), src);
+      __syncthreads();
+    }
   }
 
   float *Bo = g.B + (size_t)sl * g.npd * g.npd;
@@ -469,6 +499,7 @@ void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStr
   g.n_it = e.Np / 256; g.n_ct = (e.npd + tn - 1) / tn;
   g.sg = sb < kG1SlotGroup ? sb : kG1SlotGroup;
   g.do_q = (what & (kTermF | kTermD2vY)) ? 1 : 0;
+  g.N = e.N; g.n = e.n;
   if (what & kTermD2vR) {
     // gamma_i = X_i . (f_ups r_s) = sum_c r_s[c] X~^T[c][i] and delta_i = Ft_i . (r_s / f_psi)
     // = sum_c r_s[c] Y^T[c][i], in FP64 on the FP64 operand copies
@@ -486,20 +517,26 @@ void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStr
   launch_weights_bf16(e, sb, g.n_ct, 2, what, st);
 }
 
-void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st) {
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, int mb) {
   Gemm2RArgs g;
   const size_t NX = (size_t)e.Np * e.npd;
   g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
   g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B_f;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   const int tn = RWC * wcn;
-  g.n_mt = (e.npd + 191) / 192; g.n_ct = (e.npd + tn - 1) / tn;
+  g.n_mt = (e.npd + 64 * mb - 1) / (64 * mb); g.n_ct = (e.npd + tn - 1) / tn;
+  // k slices up to the last configuration, rounded up to the loop's step of two
+  g.KT = ((e.N + 2 * TK - 1) / (2 * TK)) * 2;
+  if (g.KT > e.Np / TK) g.KT = e.Np / TK;
+  g.n = e.n;
   const int total = sb * g.n_mt * g.n_ct;
   const dim3 grid(8 * ((total + 7) / 8));
   if (wcn == 2)
-    hipLaunchKernelGGL(k_gemm2_bf16x3_rp<2>, grid, dim3(RCfg<2>::NT), 0, st, g);
+    hipLaunchKernelGGL((k_gemm2_bf16x3_rp<2, 3>), grid, dim3(RCfg<2>::NT), 0, st, g);
+  else if (mb == 4)
+    hipLaunchKernelGGL((k_gemm2_bf16x3_rp<1, 4>), grid, dim3(RCfg<1>::NT), 0, st, g);
   else
-    hipLaunchKernelGGL(k_gemm2_bf16x3_rp<1>, grid, dim3(RCfg<1>::NT), 0, st, g);
+    hipLaunchKernelGGL((k_gemm2_bf16x3_rp<1, 3>), grid, dim3(RCfg<1>::NT), 0, st, g);
 }
 
 }  // namespace tdscha

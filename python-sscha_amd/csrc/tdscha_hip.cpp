@@ -726,7 +726,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   const int what = kTermF | kTermD2vR | kTermD2vY;
   int best = e.s_batch, best_v = 0;
   double best_t = 1e300;
-  const int n_variants = precision == kBF16x3 ? 6 : 1;
+  const int n_variants = precision == kBF16x3 ? 7 : 1;
   for (int v = 0; v < n_variants && !rc; ++v) {
     e.bf_variant = v;
     for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
@@ -770,7 +770,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 5 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 6 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

@@ -125,7 +125,8 @@ struct Ensemble {
                               // 1 = 256 x 256, 2 = 256 x 256 with direct global->LDS copies,
                               // 3 = 256/192 x 192 with wave-private A operands,
                               // 4 = 256/192 x 96, the same with two workgroups per CU,
-                              // 5 = GEMM1 of form 4 with GEMM2 of form 3
+                              // 5 = GEMM1 of form 4 with GEMM2 of form 3,
+                              // 6 = form 4 with 256 x 96 GEMM2 tiles
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
@@ -234,7 +235,8 @@ void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, 
 void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st);
 // fp64 [rows x cols] -> fragment-tiled bf16 hi / lo planes (dst, dst + rows * cols)
 void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st);
-void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st);
+// GEMM2 row tiles of 64 mb rows (mb = 3 or, with wcn = 1, 4)
+void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, int mb = 3);
 // V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
 void launch_fav(const Ensemble &e, int sb, hipStream_t st);
 // GEMM1: per image (config i, symmetry s) the quadratic forms

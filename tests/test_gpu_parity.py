@@ -120,20 +120,23 @@ def test_group_tiled_rotation_widths(precision, max_block, seed):
 
 @pytest.mark.parametrize("n,N,seed", [(150, 300, 31), (300, 520, 32), (45, 256, 33)])
 def test_bf16x3_tile_shapes(n, N, seed):
-    """The six contraction kernel forms against the oracle and against each other. Forms 0-2
-    share their arithmetic and differ only in the order of GEMM1's FP64 row sums (1e-12);
-    so do forms 3-5 among themselves. Forms 3-5 add each lane's six alpha / beta products in
+    """The seven contraction kernel forms against the oracle and against each other. Forms
+    0-2 share their arithmetic and differ only in the order of GEMM1's FP64 row sums (1e-12);
+    so do forms 3-6 among themselves. Forms 3-6 add each lane's six alpha / beta products in
     FP32 and take gamma / delta from an FP64 product, so they agree with forms 0-2 to the
-    FP32 rounding of those partial sums (1e-6 relative)."""
+    FP32 rounding of those partial sums (1e-6 relative). Form 6 runs form 4's GEMM1 and, in
+    its taller GEMM2 tiles, the same MFMA sequence per output element: form 4 bit for bit."""
     ens = fixtures.random_ensemble(seed, n, N, 4, max_block=8)
     R1, Y1 = fixtures.random_perturbation(seed + 1, n)
     dev = _dev_ens(ens, precisions=("fp64", "bf16x3"))
     out = []
-    for v in (0, 1, 2, 3, 4, 5):
+    for v in (0, 1, 2, 3, 4, 5, 6):
         dev.set_tuning(v, 0)
         assert dev.tuning()[0] == v
         out.append(_check(ens, R1, Y1, 1, dev=dev, tol=2e-4, precision="bf16x3"))
-    for ref, group, tol in ((0, (1, 2), 1e-12), (3, (4, 5), 1e-12), (0, (3,), 1e-6)):
+    for a, b in zip(out[6], out[4]):
+        assert np.array_equal(a, b)
+    for ref, group, tol in ((0, (1, 2), 1e-12), (3, (4, 5, 6), 1e-12), (0, (3,), 1e-6)):
         for k in group:
             for a, b in zip(out[ref], out[k]):
                 assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(a)), (ref, k)
