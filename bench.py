@@ -30,6 +30,7 @@ rotation (48) and averages the result over the 216 translations; it is checked o
 the full operator on the same vectors (max_rel_diff_vs_full_operator).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -229,6 +230,9 @@ def main():
                          "(a kernel of known traffic for hardware-counter calibration)")
     ap.add_argument("--no-fp64", action="store_true",
                     help="skip the plain FP64 comparison application")
+    ap.add_argument("--sustained-seconds", type=float, default=8.0,
+                    help="seconds of the pure MFMA stream that measures the sustained MFMA rate "
+                         "under the board's power limit, per precision (0: skip)")
     ap.add_argument("--no-gamma", action="store_true",
                     help="skip the zone-centre (translation-invariant) operator line")
     args = ap.parse_args()
@@ -381,6 +385,32 @@ def main():
         log("fp64 plain: %.4g L/s (%.2f s) -> speedup %.2fx; groups %s" % (
             v64, t64, value / v64, g64))
         d64.close()
+    if args.sustained_seconds > 0:
+        # the sustained MFMA rate of this board under its power limit (random operands on
+        # every CU), beside the published peak: issued MFMA work over it is the share of
+        # what the power budget allows
+        def stream(prec):
+            tf, ms = ctypes.c_double(), ctypes.c_double()
+            L.check(L.lib().tdscha_mfma_stream(L.PRECISIONS[prec], float(args.sustained_seconds),
+                                               ctypes.byref(tf), ctypes.byref(ms)),
+                    "tdscha_mfma_stream")
+            log("sustained %s MFMA stream: %.1f TFLOP/s (%.2f ms per launch), ended at "
+                "%.1f (epoch s)" % (prec, tf.value, ms.value, time.time()))
+            return tf.value
+        mfma_prec = "bf16x3" if args.precision == "bf16x3" else "fp64"
+        issue = 3.0 if args.precision == "bf16x3" else 1.0   # MFMAs per product term
+        if args.precision in ("bf16x3", "fp64"):
+            sp = stream(mfma_prec)
+            for rl, grp in ((roofline, dom), (roofline_other, "gemm2" if dom == "gemm1" else
+                                              "gemm1")):
+                rl["sustained_mfma_tflops"] = round(sp, 1)
+                rl["frac_executed_sustained"] = round(rl["achieved"] * executed[grp] * issue / sp,
+                                                      4)
+        if fp64_ref is not None:
+            sp64 = stream("fp64")
+            for grp, r in fp64_ref["roofline"].items():
+                r["sustained_mfma_tflops"] = round(sp64, 1)
+                r["frac_executed_sustained"] = round(r["achieved"] * executed[grp] / sp64, 4)
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(ens)

@@ -173,6 +173,15 @@ int tdscha_jl_get_perturb_f_averages_sym(const double *X, const double *Y, const
                                          int n_configs, int n_syms, int n_blocks, int d_max,
                                          double *f_average);
 
+/* ---- Measurement -------------------------------------------------------------------
+ * No counterpart in the reference. The sustained MFMA rate of the current device: a stream
+ * of independent MFMAs with pseudo-random operands on every CU for `seconds`, timed over
+ * the second half (after the clock has settled under the board's power limit).
+ * precision TDSCHA_PREC_BF16X3 (v_mfma_f32_16x16x32_bf16) or TDSCHA_PREC_FP64
+ * (v_mfma_f64_16x16x4_f64); *tflops receives the rate, *ms_per_launch (may be NULL) the
+ * time of one launch. Synchronous. */
+int tdscha_mfma_stream(int precision, double seconds, double *tflops, double *ms_per_launch);
+
 /* No counterpart in the reference. For callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same
  * arguments, contract and FP64 result as tdscha_get_perturb_average_sym, computed with no
@@ -252,11 +261,12 @@ int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, 
 /* The tuning the autotuner sets, set or read directly: bf16_tiles 0 (256/192 x 128
  * contraction tiles), 1 (256 x 256), 2 (256 x 256, operands copied straight into
  * double-buffered LDS), 3 (256/192 x 192, A operands in registers, only B through LDS),
- * 4 (the same with 96-column tiles, two workgroups per CU) or 5 (GEMM1 of 4, GEMM2 of 3);
- * pass_slots image slots per pass (0 = all s_batch).
+ * 4 (the same with 96-column tiles, two workgroups per CU), 5 (GEMM1 of 4, GEMM2 of 3) or
+ * 6 (GEMM1 of 4, 256 x 96 GEMM2 tiles); pass_slots image slots per pass (0 = all s_batch).
  * Results do not depend on pass_slots. Forms 0-2 differ from each other only in the order of
- * GEMM1's FP64 row sums, as do forms 3-5; forms 3-5 add each lane's row-sum products in FP32
- * and form gamma / delta in an FP64 product (agreement with forms 0-2: ~1e-7 relative). */
+ * GEMM1's FP64 row sums, as do forms 3-6 (6 equals 4 bit for bit); forms 3-6 add each lane's
+ * row-sum products in FP32 and form gamma / delta in an FP64 product (agreement with forms
+ * 0-2: ~1e-7 relative). */
 int tdscha_ensemble_set_tuning(tdscha_ensemble *ens, int bf16_tiles, int pass_slots);
 int tdscha_ensemble_get_tuning(const tdscha_ensemble *ens, int *bf16_tiles, int *pass_slots);
 

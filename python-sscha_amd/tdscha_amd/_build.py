@@ -17,7 +17,7 @@ TORCH_OPS_PATH = os.path.join(LIB_DIR, "libtdscha_torch.so")
 
 SOURCES = ["tdscha_hip.cpp", "rotate.hip", "gemm.hip", "gemm_bf16.hip", "gemm_bf16_rp.hip", "operator.hip",
            "lanczos.hip", "project.hip", "symmetry.hip", "perturb.hip", "images.hip",
-           "projector.hip"]
+           "projector.hip", "mfma_stream.hip"]
 ARCH = os.environ.get("TDSCHA_OFFLOAD_ARCH", "gfx950")
 # Packed-FP32 instructions are disabled for every device pass.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -50,6 +50,9 @@ SIGNATURES = {
         _dpp, _ip, _ipp, _ip, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "tdscha_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "tdscha_drop_in_cache": (ctypes.c_int, [ctypes.c_int]),
+    "tdscha_mfma_stream": (ctypes.c_int, [ctypes.c_int, ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]),
     "tdscha_jl_get_perturb_averages_sym": (ctypes.c_int, _JL + [_dp, _dp]),
     "tdscha_jl_get_perturb_d2v_averages_sym": (ctypes.c_int, _JL + [_dp]),
     "tdscha_jl_get_perturb_f_averages_sym": (ctypes.c_int, _JL + [_dp]),
