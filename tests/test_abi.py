@@ -95,3 +95,12 @@ def test_torch_operators_load_and_declare_their_schemas():
     assert "Tensor[] psis" in schemas["apply_full_L_batched"]
     assert "Tensor grad" in schemas["force_grad"]
     assert "bool adjoint" in schemas["projector_apply"]
+
+
+def test_mfma_stream_rejects_bad_arguments_without_a_device():
+    """tdscha_mfma_stream checks its arguments before touching the device."""
+    from tdscha_amd import _lib as L
+    tf = ctypes.c_double()
+    assert L.lib().tdscha_mfma_stream(L.PREC_FP32, 1.0, ctypes.byref(tf), None) == 1
+    assert L.lib().tdscha_mfma_stream(L.PREC_BF16X3, 0.0, ctypes.byref(tf), None) == 1
+    assert L.lib().tdscha_mfma_stream(L.PREC_FP64, 1.0, None, None) == 1

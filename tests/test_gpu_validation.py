@@ -44,3 +44,19 @@ def test_bad_output_buffers_are_refused():
     ref = dev.apply_full_L(psi).clone()
     dev.apply_full_L(psi, out=good)
     assert torch.equal(good, ref)
+
+
+def test_mfma_stream_rates_are_plausible():
+    """The sustained-rate probe returns a rate between a quarter of and the published peak
+    (bf16 2500, fp64 78.6 TFLOP/s) on a short run."""
+    import ctypes
+    import torch
+    from tdscha_amd import _lib as L
+    torch.cuda.init()
+    for prec, peak in ((L.PREC_BF16X3, 2500.0), (L.PREC_FP64, 78.6)):
+        tf, ms = ctypes.c_double(), ctypes.c_double()
+        L.check(L.lib().tdscha_mfma_stream(prec, 0.6, ctypes.byref(tf), ctypes.byref(ms)),
+                "tdscha_mfma_stream")
+        print("precision %d: %.1f TFLOP/s, %.2f ms per launch" % (prec, tf.value, ms.value),
+              flush=True)
+        assert 0.25 * peak < tf.value <= 1.02 * peak
