@@ -597,6 +597,82 @@ class Lanczos(object):
                           np.asarray(b, dtype=TYPE_DP)) for a, b in pairs)
         self._set_two_phonon(self.pols.T @ M @ self.pols, " RAMAN")
 
+    def prepare_unpolarized_raman_FT(self, index=0, debug=False, eq_raman_tns=None,
+                                     use_symm=True, ens_av_raman=None, raman_tns_ens=None,
+                                     add_2ph=True):
+        """Component `index` (0-6) of the unpolarized Raman signal with configuration-dependent
+        Raman tensors (Py:1222-1336): prepare_anharmonic_raman_FT on (xx + yy + zz), (xx - yy),
+        (xx - zz), (yy - zz), xy, xz, yz; debug saves the second-order tensor under the
+        component's name, as the reference does. The tensors are arguments (the reference also
+        requires the dynamical matrix to carry an equilibrium Raman tensor, which it does not
+        use here)."""
+        if index not in range(7):
+            raise ValueError('{} should be in {}'.format(index, list(range(7))))
+        e = np.eye(3)
+        z = np.zeros(3)
+        # (pol_in, pol_out, mixed, pol_in_2, pol_out_2, pol_in_3, pol_out_3, file name)
+        table = {0: (e[0], e[0], True, e[1], e[1], e[2], e[2], "xx_plus_yy_plus_zz"),
+                 1: (e[0], e[0], True, -e[1], e[1], z, z, "xx_minus_yy"),
+                 2: (e[0], e[0], True, -e[2], e[2], z, z, "xx_minus_zz"),
+                 3: (e[1], e[1], True, -e[2], e[2], z, z, "yy_minus_zz"),
+                 4: (e[0], e[1], False, None, None, None, None, "xy_square"),
+                 5: (e[0], e[2], False, None, None, None, None, "xz_square"),
+                 6: (e[1], e[2], False, None, None, None, None, "yz_square")}
+        pi, po, mixed, pi2, po2, pi3, po3, name = table[index]
+        self.prepare_anharmonic_raman_FT(raman=raman_tns_ens, raman_eq=eq_raman_tns,
+                                         pol_in=pi, pol_out=po, mixed=mixed, pol_in_2=pi2,
+                                         pol_out_2=po2, pol_in_3=pi3, pol_out_3=po3,
+                                         add_two_ph=add_2ph, symmetrize=use_symm,
+                                         ensemble=ens_av_raman, save_raman_tensor2=debug,
+                                         file_raman_tensor2=name)
+
+    def prepare_two_ph(self, a, b):
+        """Two-phonon perturbation on the mode pair (a, b) (Py:2035-2087, Wigner only): psi
+        is reset and holds a'(1) = -sqrt(-chi^-/2) o D, b'(1) = sqrt(chi^+/2) o D with
+        D_ab = D_ba = 1/2 (D_aa = 1); the modulus is taken with the Wigner mask.
+        Deviation: an index equal to n_modes is refused (the reference's check lets it through
+        to an IndexError)."""
+        if not self.use_wigner:
+            raise NotImplementedError('The two phonon response is available only in Wigner')
+        if not (0 <= a < self.n_modes and 0 <= b < self.n_modes):
+            raise ValueError('The a-b indices must be smaller than {}'.format(self.n_modes))
+        if self.verbose:
+            print('PREPARE THE TWO PHONON PERTUBATION')
+            print('Indices selected a = {} b = {}'.format(a, b))
+        self.reset()
+        self.psi = np.zeros(self.n_modes + self.n_modes * (self.n_modes + 1), dtype=TYPE_DP)
+        D = np.zeros((self.n_modes, self.n_modes))
+        D[a, b] = 0.5
+        D[b, a] += 0.5
+        self._set_two_phonon(D, "")
+
+    def set_max_frequency(self, freq):
+        """Py:2167-2192: run the Lanczos on 4 freq^2 - L (reverse_L with shift 4 freq^2), whose
+        largest eigenvalues are the lowest of L."""
+        self.shift_value = 4 * freq * freq
+        self.reverse_L = True
+        if self.verbose:
+            print("Shift value:", self.shift_value)
+
+    def build_lanczos_matrix_from_coeffs(self, use_arnoldi=False):
+        """The tridiagonal matrix of the run (Py:4571-4612): a on the diagonal, b below and c
+        above (b above too for a symmetric run without c), mapped back through reverse_L and
+        shift_value, so that it represents L itself. This build keeps no Arnoldi matrix:
+        use_arnoldi=True raises."""
+        if use_arnoldi:
+            raise NotImplementedError("no Arnoldi matrix is kept by this build's Lanczos")
+        N = len(self.a_coeffs)
+        M = np.zeros((N, N), dtype=TYPE_DP)
+        for i in range(N):
+            M[i, i] = self.a_coeffs[i]
+            if i >= 1:
+                c = self.c_coeffs[i - 1] if len(self.c_coeffs) == len(self.b_coeffs) else \
+                    self.b_coeffs[i - 1]
+                M[i - 1, i] = c
+                M[i, i - 1] = self.b_coeffs[i - 1]
+        sign = -1 if self.reverse_L else 1
+        return sign * M - sign * np.eye(N) * self.shift_value
+
     # ------------------------------------------------------------------ psi helpers
     def get_Y1(self, half_off_diagonal=False):
         """Py:2827-2858."""
@@ -621,6 +697,51 @@ class Lanczos(object):
             R *= 0.5
             R[np.diag_indices(n)] *= 2
         return R
+
+    def _wigner_pair_factors(self):
+        """Per independent pair (row i, column j >= i, in psi's packing order): w_i w_j, X =
+        (1 + 2 n_i)(1 + 2 n_j) / 8, sqrt(-chi^-/2) and sqrt(chi^+/2) (Py:2747-2770)."""
+        n = self.n_modes
+        iu, ju = np.triu_indices(n)
+        w = np.asarray(self.w, dtype=TYPE_DP)
+        occ = 1 / (np.exp(w * RY_TO_K / self.T) - 1) if self.T > 0 else np.zeros(n)
+        wa, wb, na, nb = w[ju], w[iu], occ[ju], occ[iu]
+        X = (1 + 2 * na) * (1 + 2 * nb) / 8
+        chi_m = (wa - wb) * (na - nb) / (2 * wa * wb)
+        chi_p = (wa + wb) * (1 + na + nb) / (2 * wa * wb)
+        return wa * wb, X, np.sqrt(np.maximum(-0.5 * chi_m, 0.0)), np.sqrt(0.5 * chi_p)
+
+    def _unpack_symmetric(self, v):
+        n = self.n_modes
+        M = np.zeros((n, n), dtype=TYPE_DP)
+        M[np.triu_indices(n)] = v
+        return M + np.triu(M, 1).T
+
+    def get_alpha1_beta1_wigner(self, get_alpha=True):
+        """Upsilon^(1) = alpha^(1) (get_alpha) or beta^(1) as a symmetric (n, n) matrix from
+        the a'(1), b'(1) sectors of psi (Py:2713-2822):
+        alpha = w_i w_j / X (sqrt(-chi^-/2) a' - sqrt(chi^+/2) b'),
+        beta = 1 / X (sqrt(-chi^-/2) a' + sqrt(chi^+/2) b')."""
+        n = self.n_modes
+        nw2 = n * (n + 1) // 2
+        psi = np.asarray(self.psi, dtype=TYPE_DP)
+        a_all, b_all = psi[n:n + nw2], psi[n + nw2:n + 2 * nw2]
+        ww, X, sm, sp = self._wigner_pair_factors()
+        if get_alpha:
+            return self._unpack_symmetric((ww / X) * (sm * a_all - sp * b_all))
+        return self._unpack_symmetric((sm * a_all + sp * b_all) / X)
+
+    def get_a1_b1_wigner(self, get_a1=True):
+        """The inverse change of variables (Py:2630-2710), from get_alpha1_beta1_wigner's
+        matrices back to packed vectors: sqrt(-chi^-/2) a'(1) = X / 2 (alpha / (w_i w_j) +
+        beta) (get_a1), or b'(1) = -X / 2 (alpha / (w_i w_j) - beta) / sqrt(chi^+/2)."""
+        iu = np.triu_indices(self.n_modes)
+        alpha = self.get_alpha1_beta1_wigner(True)[iu]
+        beta = self.get_alpha1_beta1_wigner(False)[iu]
+        ww, X, _sm, sp = self._wigner_pair_factors()
+        if get_a1:
+            return 0.5 * X * (alpha / ww + beta)
+        return -0.5 * X * (alpha / ww - beta) / sp
 
     def mask_dot_wigner(self, debug=False):
         """Py:5790-5839: weight 2 on off-diagonal pair entries."""

@@ -216,6 +216,34 @@ def test_anharmonic_raman_FT_matches_oracle(tmp_path):
     assert rel(lz.psi[:n], full[:n]) < 1e-15          # the one-phonon sector is left alone
 
 
+def test_unpolarized_raman_FT_components():
+    """The seven components of the unpolarized signal with configuration-dependent tensors
+    (Py:1222-1336): (xx + yy + zz), (xx - yy), (xx - zz), (yy - zz), xy, xz, yz, each against
+    the oracle's averages composed for that component's polarization pairs."""
+    from tdscha_amd import perturbations as pt
+    print("case: prepare_unpolarized_raman_FT", flush=True)
+    lz, ens, cr, rots, irt, rng = _rock_salt_lanczos(seed=15)
+    N, nat = ens.u_disps.shape[:2]
+    raman = rng.standard_normal((N, 3, 3, 3 * nat))
+    raman_eq = rng.standard_normal((3, 3, 6))
+    d1 = orc.d1alpha_dR_av(ens.rho, raman)
+    ub = ens.u_disps.reshape(N, -1) * pt.A_TO_BOHR
+    d2 = orc.d2alpha_dR_av(ub, ens.rho, raman - np.tile(raman_eq, 8)[None], _ups(lz))
+    sm = 1 / np.sqrt(lz.m)
+    d2 = d2 * sm[None, None, :, None] * sm[None, None, None, :]
+    x, y, z = np.eye(3)
+    comps = {0: [(x, x), (y, y), (z, z)], 1: [(x, x), (-y, y)], 2: [(x, x), (-z, z)],
+             3: [(y, y), (-z, z)], 4: [(x, y)], 5: [(x, z)], 6: [(y, z)]}
+    for index, pairs in comps.items():
+        lz.prepare_unpolarized_raman_FT(index, eq_raman_tns=raman_eq, use_symm=False,
+                                        ens_av_raman=ens, raman_tns_ens=raman, add_2ph=True)
+        v = sum(pt.raman_vector(d1, a, b) for a, b in pairs)
+        M = sum(np.einsum("abcd, a, b -> cd", d2, a, b) for a, b in pairs)
+        assert rel(lz.psi, _oracle_psi(lz, v, M, pt)) < 1e-11, index
+    with pytest.raises(ValueError):
+        lz.prepare_unpolarized_raman_FT(7, raman_tns_ens=raman, ens_av_raman=ens)
+
+
 def test_anharmonic_raman_FT_symmetrized_is_raman_inactive():
     """Rock salt has no first-order Raman activity: with the space-group average the
     one-phonon sector vanishes while the two-phonon sectors do not."""

@@ -76,3 +76,95 @@ def test_continued_fraction_host_matches_oracle():
             g_o = orc.green_function_continued_fraction(st["a"], st["b"], st["c"], w, term, 1,
                                                         1e-5, wig)
             assert np.allclose(g, g_o, rtol=1e-12, atol=0)
+
+
+def test_wigner_alpha_beta_and_their_inverse(mode10):
+    """get_alpha1_beta1_wigner against the oracle's restatement (pinned through the full L on
+    the reference's Wigner Krylov vectors), and get_a1_b1_wigner as its inverse: it must give
+    back b'(1) exactly and sqrt(-chi^-/2) a'(1), at T > 0 and at T = 0 (chi^- = 0)."""
+    for T in (300.0, 0.0):
+        lz = _lanczos(mode10)
+        lz.T = T
+        lz.use_wigner = True
+        n = lz.n_modes
+        nw2 = n * (n + 1) // 2
+        lz.psi = np.random.default_rng(11).standard_normal(n + 2 * nw2)
+        alpha = lz.get_alpha1_beta1_wigner(True)
+        ref = orc.wigner_alpha1(lz.psi, mode10.w, T)
+        assert np.allclose(alpha, ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+        assert np.array_equal(alpha, alpha.T)
+        beta = lz.get_alpha1_beta1_wigner(False)
+        assert np.array_equal(beta, beta.T)
+        b1 = lz.get_a1_b1_wigner(False)
+        b_true = lz.psi[n + nw2:]
+        assert np.allclose(b1, b_true, rtol=1e-12, atol=1e-12 * np.abs(b_true).max())
+        iu = np.triu_indices(n)
+        chi_m, _ = orc.chi_minus_plus(mode10.w, T)
+        a1 = lz.get_a1_b1_wigner(True)
+        a_scaled = np.sqrt(-0.5 * chi_m[iu[1], iu[0]]) * lz.psi[n:n + nw2]
+        # at T = 0 chi^- vanishes: a1 is the rounding of the cancelling b' terms
+        scale = max(np.abs(a_scaled).max(), np.abs(b_true).max())
+        assert np.max(np.abs(a1 - a_scaled)) <= 1e-12 * scale
+        if T > 0:
+            assert np.abs(a_scaled).max() > 1e-3 * scale
+
+
+def test_prepare_two_ph(mode10):
+    """The two-phonon perturbation on a mode pair equals the oracle's two-phonon sectors of
+    D = (E_ab + E_ba) / 2 (pinned by closed forms), with the Wigner-mask modulus."""
+    from tdscha_amd import DynamicalLanczos as DL
+    lz = _lanczos(mode10)
+    lz.verbose = False
+    with pytest.raises(NotImplementedError):
+        lz.prepare_two_ph(1, 2)
+    lz.use_wigner = True
+    n = lz.n_modes
+    for a, b in ((3, 7), (5, 5)):
+        lz.prepare_two_ph(a, b)
+        D = np.zeros((n, n))
+        D[a, b] += 0.5
+        D[b, a] += 0.5
+        want = orc.pack_two_phonon(np.zeros(n + n * (n + 1)),
+                                   *orc.two_phonon_sectors(D, mode10.w, mode10.T))
+        assert np.allclose(lz.psi, want, rtol=0, atol=1e-15 * np.abs(want).max())
+        assert not np.any(lz.psi[:n])
+        mask = orc.mask_dot_wigner(n)
+        assert np.isclose(lz.perturbation_modulus, want.dot(want * mask), rtol=1e-14)
+    with pytest.raises(ValueError):
+        lz.prepare_two_ph(n, 0)
+
+
+def _dense_L(ens):
+    n_psi = orc.psi_len(len(ens.w))
+    return np.array([orc.apply_full_L(ens, v) for v in np.eye(n_psi)]).T
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_lanczos_matrix_from_coefficients_has_the_spectrum_of_L(reverse):
+    """On a complete Krylov space (2 modes, 8-dimensional psi) the matrix that
+    build_lanczos_matrix_from_coeffs makes from a full-GS run's a, b, c has the eigenvalues
+    of L; with set_max_frequency the run is made on 4 f^2 - L and the matrix maps back."""
+    from tdscha_amd import DynamicalLanczos as DL
+    ens = fixtures.random_ensemble(13, 2, 12, 2, max_block=2)
+    Ld = _dense_L(ens)
+    want = np.sort(np.linalg.eigvals(Ld).real)
+    lz = DL.Lanczos(mode=DL.MODE_GPU_FP64)
+    lz.verbose = False
+    op = Ld
+    if reverse:
+        lz.set_max_frequency(float(np.max(ens.w)))
+        op = -Ld + lz.shift_value * np.eye(len(Ld))
+    psi0 = np.random.default_rng(2).standard_normal(len(Ld))
+    run = orc.run_FT(lambda q: op @ q, lambda p: op.T @ p, psi0, len(Ld), full_gs=True)
+    lz.a_coeffs, lz.b_coeffs, lz.c_coeffs = list(run.a), list(run.b), list(run.c)
+    T = lz.build_lanczos_matrix_from_coeffs()
+    assert T.shape == (len(run.a), len(run.a))
+    got = np.linalg.eigvals(T).real
+    # every eigenvalue of T is one of L's, and every distinct eigenvalue of L appears
+    scale = np.max(np.abs(want))
+    distinct = [want[0]] + [x for p, x in zip(want, want[1:]) if x - p > 1e-8 * scale]
+    assert len(got) == len(distinct)
+    assert max(np.min(np.abs(want - x)) for x in got) <= 1e-8 * scale
+    assert max(np.min(np.abs(got - x)) for x in distinct) <= 1e-8 * scale
+    with pytest.raises(NotImplementedError):
+        lz.build_lanczos_matrix_from_coeffs(use_arnoldi=True)
