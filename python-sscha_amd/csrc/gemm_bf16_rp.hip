@@ -136,6 +136,28 @@ __global__ void k_tile_bf16x3(const double *__restrict__ src, int rows, int cols
   }
 }
 
+// FP32 [rows x cols] row-major -> accumulator-tiled copy (rows % 16 == 0, cols % 16 == 0):
+// 16 x 16 block (rb, cb) holds, at lane l = lr + 16 lg of a wave, the four values of rows
+// 16 rb + 4 lg .. + 4 in column 16 cb + lr as one 16-byte chunk, so that the row-sum operands
+// of one accumulator block are a single 1 KB wave load (in the row-major copy they are four
+// scattered 4-byte loads per lane).
+__global__ void k_tile_acc_f32(const float *__restrict__ src, int rows, int cols,
+                               float *__restrict__ dst) {
+  const int ncb = cols / 16;
+  const size_t n4 = (size_t)rows * cols / 4;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const int lane = (int)(i & 63);
+    const size_t blk = i >> 6;
+    const int rb = (int)(blk / ncb), cb = (int)(blk % ncb);
+    const size_t o = (size_t)(16 * rb + 4 * (lane >> 4)) * cols + 16 * cb + (lane & 15);
+    v4f v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = src[o + (size_t)r * cols];
+    *reinterpret_cast<v4f *>(dst + 4 * i) = v;
+  }
+}
+
 // ---- fragments and MFMAs
 struct BFrag {
   v8bf h, l;
@@ -213,7 +235,7 @@ __device__ inline Halves live_halves(int mi, int nj) {
 struct Gemm1RArgs {
   const uint16_t *Xh, *Xl;    // [Np x npd] fragment-tiled
   const uint16_t *Yh, *Yl;    // [s_batch x npd x npd] planes of the rotated Y1
-  const float *Xf, *Ftf;      // [Np x npd] epilogue operands in FP32
+  const float *Xa, *Fta;      // [Np x npd] epilogue operands in FP32, accumulator-tiled
   double *P;                  // [sb x n_ct x 2 x Np] row sums per column tile
   int Np, npd, sb, n_it, n_ct, sg, do_q;   // sg: image slots per tile group
   int N, n;                   // configurations and modes (the rest is zero padding)
@@ -222,23 +244,32 @@ struct Gemm1RArgs {
 // Row sums of block row ti of this wave over its columns below nj: alpha = sum X Q and
 // beta = sum Ft Q, each lane's six products added in FP32 (Q itself carries the BF16x3
 // error), then reduced over the 16 lanes of each row group in FP64 with DPP. gamma and delta
-// do not involve Q and come from an FP64 product outside this kernel.
+// do not involve Q and come from an FP64 product outside this kernel. The twelve operand
+// chunks of the block row are loaded first, unconditionally (blocks past the last column
+// read the last block and are not added), so that their latencies overlap.
 template <int MB>
 __device__ inline void row_sums(const Gemm1RArgs &g, int ti, int row0, int cw, int nj,
                                 const v4f (&acc)[MB][RNB], double (&p)[4][2]) {
-  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const int l = threadIdx.x & 63;
+  const int ncb = g.npd / 16, rb = (row0 >> 4) + ti, cb0 = cw >> 4;
+  v4f xa[RNB], fa[RNB];
+#pragma unroll
+  for (int tj = 0; tj < RNB; ++tj) {
+    const int cb = min(cb0 + tj, ncb - 1);
+    const size_t o = (((size_t)rb * ncb + cb) * 64 + l) * 4;
+    xa[tj] = *reinterpret_cast<const v4f *>(g.Xa + o);
+    fa[tj] = *reinterpret_cast<const v4f *>(g.Fta + o);
+  }
+  // blocks at or past nj add x * 0 (the sums keep their values; no branch keeps the loads
+  // above from being sunk into one)
   float pa[4] = {0.f, 0.f, 0.f, 0.f}, pb[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int tj = 0; tj < RNB; ++tj) {
-    if (tj < nj) {
-      const int c = cw + tj * 16 + lr;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const size_t o = (size_t)(row0 + ti * 16 + 4 * lg + r) * g.npd + c;
-        const float q = acc[ti][tj][r];
-        pa[r] = fmaf(g.Xf[o], q, pa[r]);
-        pb[r] = fmaf(g.Ftf[o], q, pb[r]);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const float q = tj < nj ? acc[ti][tj][r] : 0.f;
+      pa[r] = fmaf(xa[tj][r], q, pa[r]);
+      pb[r] = fmaf(fa[tj][r], q, pb[r]);
     }
   }
 #pragma unroll
@@ -487,13 +518,20 @@ void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hi
   hipLaunchKernelGGL(k_tile_bf16x3, dim3((unsigned)grid), dim3(256), 0, st, src, rows, cols, dst);
 }
 
+void launch_tile_acc_f32(const float *src, int rows, int cols, float *dst, hipStream_t st) {
+  const size_t n4 = (size_t)rows * cols / 4;
+  size_t grid = (n4 + 255) / 256;
+  if (grid > 65535u * 16u) grid = 65535u * 16u;
+  hipLaunchKernelGGL(k_tile_acc_f32, dim3((unsigned)grid), dim3(256), 0, st, src, rows, cols, dst);
+}
+
 void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st) {
   Gemm1RArgs g;
   const size_t plane = (size_t)e.s_batch * e.npd * e.npd;
   const size_t NX = (size_t)e.Np * e.npd;
   g.Xh = e.X_frag; g.Xl = e.X_frag + NX;
   g.Yh = e.Ys_bf; g.Yl = e.Ys_bf + plane;
-  g.Xf = e.X_f; g.Ftf = e.Ft_f; g.P = e.Pw;
+  g.Xa = e.X_acc; g.Fta = e.Ft_acc; g.P = e.Pw;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
   const int tn = RWC * wcn;
   g.n_it = e.Np / 256; g.n_ct = (e.npd + tn - 1) / tn;

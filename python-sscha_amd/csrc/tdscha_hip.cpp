@@ -539,6 +539,10 @@ int ensemble_create_impl(tdscha_ensemble **out, int device, const double *X, con
   if (want32 || wantbf) rc |= dev_alloc(h, &e.X_f, NX);
   if (wantbf) rc |= dev_alloc(h, &e.Ft_f, NX);
   if (wantbf) {
+    rc |= dev_alloc(h, &e.X_acc, NX);
+    rc |= dev_alloc(h, &e.Ft_acc, NX);
+  }
+  if (wantbf) {
     rc |= dev_alloc(h, &e.X_hi, 2 * NX);
     rc |= dev_alloc(h, &e.XtT_hi, 2 * NX);
     rc |= dev_alloc(h, &e.X_frag, 2 * NX);
@@ -594,6 +598,8 @@ int ensemble_create_impl(tdscha_ensemble **out, int device, const double *X, con
     launch_convert(e.XtT, e.XtT_hi, NX, kBF16x3, nullptr);
     launch_tile_bf16x3(e.X, Np, npd, e.X_frag, nullptr);
     launch_tile_bf16x3(e.XtT, npd, Np, e.XtT_frag, nullptr);
+    launch_tile_acc_f32(e.X_f, Np, npd, e.X_acc, nullptr);
+    launch_tile_acc_f32(e.Ft_f, Np, npd, e.Ft_acc, nullptr);
   }
   hipError_t err = hipDeviceSynchronize();
   if (err == hipSuccess) err = hipGetLastError();

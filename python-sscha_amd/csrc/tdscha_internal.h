@@ -90,6 +90,9 @@ struct Ensemble {
   // then lo plane) read by the register-operand contraction kernels (tile form 3)
   uint16_t *X_frag = nullptr, *XtT_frag = nullptr;
   float *X_f = nullptr, *Ft_f = nullptr, *XtT_f = nullptr, *YT_f = nullptr;
+  // X_f and Ft_f in the accumulator-tiled layout (16 x 16 blocks of 64 lane chunks of 4
+  // values), the GEMM1 row-sum operands of tile forms 3-6
+  float *X_acc = nullptr, *Ft_acc = nullptr;
 
   // per-chain inputs / accumulators, for up to n_chains chains (grown on demand)
   int n_chains = 0;
@@ -235,6 +238,7 @@ void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, 
 void launch_gemm1_bf16x3_rp(const Ensemble &e, int sb, int what, int wcn, hipStream_t st);
 // fp64 [rows x cols] -> fragment-tiled bf16 hi / lo planes (dst, dst + rows * cols)
 void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st);
+void launch_tile_acc_f32(const float *src, int rows, int cols, float *dst, hipStream_t st);
 // GEMM2 row tiles of 64 mb rows (mb = 3 or, with wcn = 1, 4)
 void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, int mb = 3);
 // V[s] = Y^T e1_s + X~^T e2_s for the first sb slots (FP64 MFMA, split over configurations)
