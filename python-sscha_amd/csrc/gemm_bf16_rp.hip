@@ -509,6 +509,131 @@ __global__ __launch_bounds__(RCfg<WCN>::NT, 2 / WCN) void k_gemm2_bf16x3_rp(Gemm
       }
 }
 
+// ---- GEMM2 with eight row waves sharing one formed operand panel (tile form 7). The formed
+// operand is the kernel's heaviest load stream (its two FP32 sources and the weights: more
+// vector-memory instructions per slice than the A fragments), and the 96-column panel formed
+// per slice serves every wave of the workgroup: eight waves stacked along the rows (TM =
+// 128 MB rows, one workgroup per CU) halve the formation loads per MFMA against four. Each
+// wave runs the MFMA sequence of k_gemm2_bf16x3_rp (same slices, same order): the stored
+// B_s are those of tile forms 4 and 6 bit for bit.
+constexpr int kR8Threads = 512;
+
+// The 96 x 32 slice as 768 items (row, four k): thread t forms item t (rows 0-63) and, in
+// waves 0-3, item t + 512 (rows 64-95); both share the weights of k quad t % 8.
+struct GSrc8 {
+  v4f ya, xa, yb, xb, wa, wb;
+};
+
+__device__ inline void rload_g8(const Gemm2RArgs &g, const float *__restrict__ wf, int c0,
+                                int k0, bool two, GSrc8 &s) {
+  const int t = threadIdx.x, kk = k0 + (t & 7) * 4;
+  int ra = c0 + (t >> 3), rb = ra + 64;
+  ra = ra < g.npd ? ra : g.npd - 1;
+  rb = rb < g.npd ? rb : g.npd - 1;
+  s.ya = *reinterpret_cast<const v4f *>(g.YTf + (size_t)ra * g.Np + kk);
+  s.xa = *reinterpret_cast<const v4f *>(g.XtTf + (size_t)ra * g.Np + kk);
+  if (two) {
+    s.yb = *reinterpret_cast<const v4f *>(g.YTf + (size_t)rb * g.Np + kk);
+    s.xb = *reinterpret_cast<const v4f *>(g.XtTf + (size_t)rb * g.Np + kk);
+  }
+  s.wa = *reinterpret_cast<const v4f *>(wf + kk);
+  s.wb = *reinterpret_cast<const v4f *>(wf + g.Np + kk);
+}
+
+__device__ inline void rstore_item(RStage<RWC> &st, int row, int q, v4f wa, v4f wb, v4f y,
+                                   v4f x) {
+  uint32_t hi0, lo0, hi1, lo1;
+  split2(fmaf(wa.x, y.x, wb.x * x.x), fmaf(wa.y, y.y, wb.y * x.y), hi0, lo0);
+  split2(fmaf(wa.z, y.z, wb.z * x.z), fmaf(wa.w, y.w, wb.w * x.w), hi1, lo1);
+  const int o = row * TK + (((q >> 1) ^ rswz(row)) * 8) + (q & 1) * 4;
+  *reinterpret_cast<u32x2 *>(st.bh + o) = (u32x2){hi0, hi1};
+  *reinterpret_cast<u32x2 *>(st.bl + o) = (u32x2){lo0, lo1};
+}
+
+__device__ inline void rstore_g8(RStage<RWC> &st, bool two, const GSrc8 &s) {
+  const int t = threadIdx.x, q = t & 7;
+  rstore_item(st, t >> 3, q, s.wa, s.wb, s.ya, s.xa);
+  if (two) rstore_item(st, 64 + (t >> 3), q, s.wa, s.wb, s.yb, s.xb);
+}
+
+template <int MB>
+__global__ __launch_bounds__(kR8Threads, 1) void k_gemm2_bf16x3_r8(Gemm2RArgs g) {
+  constexpr int TM = 128 * MB;
+  __shared__ RStage<RWC> st[2];
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id % g.sb;
+  const int rest = id / g.sb;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int m0 = mt * TM, c0 = ct * RWC;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
+  const bool two = w < 4;                        // wave-uniform: waves 0-3 form two items
+  const int row0 = m0 + w * 16 * MB;
+  const int mi = clamp_blocks(g.n, row0, MB);
+  const int nj = clamp_blocks(g.n, c0, RNB);
+  const Halves hv = live_halves(mi, nj);
+  const float *wf = g.Wf + (size_t)sl * 2 * g.Np;
+
+  v4f acc[MB][RNB];
+#pragma unroll
+  for (int a = 0; a < MB; ++a)
+#pragma unroll
+    for (int b = 0; b < RNB; ++b) acc[a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
+
+  const int KT = g.KT, last = (KT - 1) * TK;
+  // fragment offsets of this lane in a stage (one column wave: rows lr of each block)
+  const int2 base = make_int2(lr * TK + ((lg ^ rswz(lr)) * 8), lr * TK + ((lg ^ rswz(lr) ^ 2) * 8));
+  const int n_ks = g.Np / TK, n_rb = g.npd / 16, rb0 = row0 / 16;
+  GSrc8 src;
+  AFrag<MB> a0, a1;
+  rload_g8(g, wf, c0, 0, two, src);
+  rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 0, a0);
+  rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, 1, a1);
+  rstore_g8(st[0], two, src);
+  rload_g8(g, wf, c0, TK, two, src);
+  __syncthreads();
+  if (hv.h0) {
+    for (int kt = 0; kt < KT; kt += 2) {
+      rmma<MB, 0, RNB / 2>(st[0], a0, base, acc);
+      rstore_g8(st[1], two, src);
+      rload_g8(g, wf, c0, min((kt + 2) * TK, last), two, src);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[0], a0, base, acc);
+      rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 2, KT - 1), a0);
+      __syncthreads();
+      rmma<MB, 0, RNB / 2>(st[1], a1, base, acc);
+      rstore_g8(st[0], two, src);
+      rload_g8(g, wf, c0, min((kt + 3) * TK, last), two, src);
+      if (hv.h1) rmma<MB, RNB / 2, RNB>(st[1], a1, base, acc);
+      rload_a<MB>(g.Ah, g.Al, n_ks, n_rb, rb0, min(kt + 3, KT - 1), a1);
+      __syncthreads();
+    }
+  } else {
+    for (int kt = 0; kt < KT; kt += 2) {
+      rstore_g8(st[1], two, src);
+      rload_g8(g, wf, c0, min((kt + 2) * TK, last), two, src);
+      __syncthreads();
+      rstore_g8(st[0], two, src);
+      rload_g8(g, wf, c0, min((kt + 3) * TK, last), two, src);
+      __syncthreads();
+    }
+  }
+
+  float *Bo = g.B + (size_t)sl * g.npd * g.npd;
+#pragma unroll
+  for (int ti = 0; ti < MB; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < RNB; ++tj)
+      if (ti < mi && tj < nj) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = row0 + ti * 16 + 4 * lg + r;
+          const int c = c0 + tj * 16 + lr;
+          Bo[(size_t)m * g.npd + c] = acc[ti][tj][r];
+        }
+      }
+}
+
 }  // namespace
 
 void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st) {
@@ -561,15 +686,18 @@ void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, 
   g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
   g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B_f;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
-  const int tn = RWC * wcn;
-  g.n_mt = (e.npd + 64 * mb - 1) / (64 * mb); g.n_ct = (e.npd + tn - 1) / tn;
+  const int tn = RWC * (wcn == 8 ? 1 : wcn);
+  const int tm = (wcn == 8 ? 128 : 64) * mb;
+  g.n_mt = (e.npd + tm - 1) / tm; g.n_ct = (e.npd + tn - 1) / tn;
   // k slices up to the last configuration, rounded up to the loop's step of two
   g.KT = ((e.N + 2 * TK - 1) / (2 * TK)) * 2;
   if (g.KT > e.Np / TK) g.KT = e.Np / TK;
   g.n = e.n;
   const int total = sb * g.n_mt * g.n_ct;
   const dim3 grid(8 * ((total + 7) / 8));
-  if (wcn == 2)
+  if (wcn == 8)
+    hipLaunchKernelGGL((k_gemm2_bf16x3_r8<4>), grid, dim3(kR8Threads), 0, st, g);
+  else if (wcn == 2)
     hipLaunchKernelGGL((k_gemm2_bf16x3_rp<2, 3>), grid, dim3(RCfg<2>::NT), 0, st, g);
   else if (mb == 4)
     hipLaunchKernelGGL((k_gemm2_bf16x3_rp<1, 4>), grid, dim3(RCfg<1>::NT), 0, st, g);

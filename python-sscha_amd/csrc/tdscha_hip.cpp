@@ -730,33 +730,70 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   TD_CHECK(hipEventCreate(&t0));
   TD_CHECK(hipEventCreate(&t1));
   const int what = kTermF | kTermD2vR | kTermD2vY;
-  int best = e.s_batch, best_v = 0;
+  // time of `reps` runs of the passes over the first s_end symmetries (after one warm-up
+  // run when asked)
+  auto timed = [&](int s_end, int reps, bool warm, double &ms_out) -> int {
+    int r = warm ? run_passes(e, precision, what, n_chains, nullptr, 0, s_end) : 0;
+    if (!r && hipEventRecord(t0, nullptr) != hipSuccess) r = 10;
+    for (int k = 0; k < reps && !r; ++k) r = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);
+    if (!r && hipEventRecord(t1, nullptr) != hipSuccess) r = 10;
+    float ms = 0.f;
+    if (!r && (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
+      r = 10;
+    ms_out = ms;
+    return r;
+  };
+  // phase 1: the slot count per pass, with the default tile form (form 4 for BF16x3): at
+  // least four passes and 168 images per chain (or all images), best of two
+  const bool bf = precision == kBF16x3;
+  e.bf_variant = bf ? 4 : 0;
+  int best = e.s_batch, best_se = 1;
   double best_t = 1e300;
-  const int n_variants = precision == kBF16x3 ? 7 : 1;
-  for (int v = 0; v < n_variants && !rc; ++v) {
-    e.bf_variant = v;
-    for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6; c /= 2, ++tried) {
-      e.pass_slots = c;
-      const int ns = c / n_chains;
-      // at least four passes and 168 images per chain (or all images), best of two timings
-      int s_end = 4 * ns > 168 ? 4 * ns : 168;
-      if (s_end > e.N_sym) s_end = e.N_sym;
-      rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);   // warm-up
-      double best_ms = 1e300;
-      for (int rep = 0; rep < 2 && !rc; ++rep) {
-        if (hipEventRecord(t0, nullptr) != hipSuccess) rc = 10;
-        if (!rc) rc = run_passes(e, precision, what, n_chains, nullptr, 0, s_end);
-        if (!rc && hipEventRecord(t1, nullptr) != hipSuccess) rc = 10;
-        float ms = 0.f;
-        if (!rc && (hipEventSynchronize(t1) != hipSuccess ||
-                    hipEventElapsedTime(&ms, t0, t1) != hipSuccess))
-          rc = 10;
-        if (!rc && ms < best_ms) best_ms = ms;
+  for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6 && !rc; c /= 2, ++tried) {
+    e.pass_slots = c;
+    const int ns = c / n_chains;
+    int s_end = 4 * ns > 168 ? 4 * ns : 168;
+    if (s_end > e.N_sym) s_end = e.N_sym;
+    double best_ms = 1e300;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+      double ms = 0.0;
+      rc = timed(s_end, 1, rep == 0, ms);
+      if (!rc && ms < best_ms) best_ms = ms;
+    }
+    if (rc) break;
+    const double per = 1e3 * best_ms / ((double)s_end * n_chains * e.N);
+    if (per < best_t) { best_t = per; best = c; best_se = s_end; }
+    if (s_end == e.N_sym && c < e.s_batch) break;   // smaller passes cannot fill more
+  }
+  // phase 2 (BF16x3): every tile form at that slot count, each timed over at least 0.8 s of
+  // passes, in two interleaved rounds, after 4 s of passes that bring the board to its
+  // sustained clock under the power limit (the forms' order there can differ from that of
+  // short bursts at a higher clock); the faster of the two rounds counts
+  int best_v = e.bf_variant;
+  if (bf && !rc) {
+    constexpr int n_variants = 8;
+    e.pass_slots = best;
+    double one = 0.0;
+    rc = timed(best_se, 1, true, one);
+    int reps = one > 0.0 ? (int)(800.0 / one) + 1 : 1;
+    if (reps > 64) reps = 64;
+    int heat = one > 0.0 ? (int)(4000.0 / one) + 1 : 1;
+    if (heat > 256) heat = 256;
+    double ignored = 0.0;
+    if (!rc) rc = timed(best_se, heat, false, ignored);
+    double vt[n_variants];
+    for (int v = 0; v < n_variants; ++v) vt[v] = 1e300;
+    for (int round = 0; round < 2 && !rc; ++round)
+      for (int v = 0; v < n_variants && !rc; ++v) {
+        e.bf_variant = v;
+        double ms = 0.0;
+        rc = timed(best_se, reps, true, ms);
+        if (!rc && ms < vt[v]) vt[v] = ms;
       }
-      if (rc) break;
-      const double per = 1e3 * best_ms / ((double)s_end * n_chains * e.N);
-      if (per < best_t) { best_t = per; best = c; best_v = v; }
-      if (s_end == e.N_sym && c < e.s_batch) break;   // smaller passes cannot fill more
+    if (!rc) {
+      for (int v = 0; v < n_variants; ++v)
+        if (vt[v] < vt[best_v]) best_v = v;
+      best_t = 1e3 * vt[best_v] / ((double)best_se * n_chains * e.N * reps);
     }
   }
   (void)hipEventDestroy(t0);
@@ -776,7 +813,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 6 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 7 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

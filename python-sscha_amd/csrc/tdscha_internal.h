@@ -129,7 +129,9 @@ struct Ensemble {
                               // 3 = 256/192 x 192 with wave-private A operands,
                               // 4 = 256/192 x 96, the same with two workgroups per CU,
                               // 5 = GEMM1 of form 4 with GEMM2 of form 3,
-                              // 6 = form 4 with 256 x 96 GEMM2 tiles
+                              // 6 = form 4 with 256 x 96 GEMM2 tiles,
+                              // 7 = GEMM1 of form 4 with 512 x 96 GEMM2 tiles of
+                              //     eight row waves sharing the formed operand
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups
