@@ -254,7 +254,7 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
  * s_batch/2, ... (at most six) and keeps the fastest per image; for BF16x3 it then times the
  * eight contraction kernel forms at that slot count (after 4 s of passes that bring the board
  * to its sustained clock, each over at least 0.8 s of passes, two interleaved rounds) and
- * keeps the fastest.
+ * keeps the fastest, or the default form 7 unless another one is faster by more than 1 %.
  * Returns
  * the chosen slot count and its time per (chain x configuration x symmetry) image in
  * microseconds (either may be NULL). Call before capturing Lanczos graphs on the ensemble. */
