@@ -58,6 +58,7 @@ BF16_KERNELS = {
     5: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<2, 3>"),
     6: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_rp<1, 4>"),
     7: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_r8<4>"),
+    8: ("k_gemm1_bf16x3_rp<1>", "k_gemm2_bf16x3_r7"),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bf16x3_rocksalt6.md")
 
@@ -224,7 +225,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile shape and pass size instead of timing them")
     ap.add_argument("--tiles", type=int, default=-1,
-                    help="BF16x3 tile form 0-7 (with --pass-slots: no autotuning)")
+                    help="BF16x3 tile form 0-8 (with --pass-slots: no autotuning)")
     ap.add_argument("--pass-slots", type=int, default=0, help="image slots per pass")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed steps, copy a buffer of this many MiB on the device "

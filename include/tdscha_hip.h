@@ -252,9 +252,9 @@ int tdscha_apply_full_L_batched(tdscha_ensemble *ens, int precision, int n_chain
 /* Autotuner (no reference counterpart): times the batched L-application for n_chains chains
  * (at least four passes and 168 symmetry images, best of two) at slot counts s_batch,
  * s_batch/2, ... (at most six) and keeps the fastest per image; for BF16x3 it then times the
- * eight contraction kernel forms at that slot count (after 4 s of passes that bring the board
+ * nine contraction kernel forms at that slot count (after 4 s of passes that bring the board
  * to its sustained clock, each over at least 0.8 s of passes, two interleaved rounds) and
- * keeps the fastest, or the default form 7 unless another one is faster by more than 1 %.
+ * keeps the fastest, or the default form 8 unless another one is faster by more than 1 %.
  * Returns
  * the chosen slot count and its time per (chain x configuration x symmetry) image in
  * microseconds (either may be NULL). Call before capturing Lanczos graphs on the ensemble. */
@@ -265,11 +265,12 @@ int tdscha_ensemble_autotune(tdscha_ensemble *ens, int precision, int n_chains, 
  * contraction tiles), 1 (256 x 256), 2 (256 x 256, operands copied straight into
  * double-buffered LDS), 3 (256/192 x 192, A operands in registers, only B through LDS),
  * 4 (the same with 96-column tiles, two workgroups per CU), 5 (GEMM1 of 4, GEMM2 of 3),
- * 6 (GEMM1 of 4, 256 x 96 GEMM2 tiles) or 7 (GEMM1 of 4, 512 x 96 GEMM2 tiles of eight
- * row waves sharing one formed operand panel, 64 rows per wave); pass_slots image slots per pass
- * (0 = all s_batch). Results do not depend on pass_slots. Forms 0-2 differ from each other
- * only in the order of GEMM1's FP64 row sums, as do forms 3-7 (6 and 7 equal 4 bit for bit:
- * the same MFMA sequence per output element); forms 3-7 add each lane's
+ * 6 (GEMM1 of 4, 256 x 96 GEMM2 tiles), 7 (GEMM1 of 4, 512 x 96 GEMM2 tiles of eight
+ * row waves sharing one formed operand panel, 64 rows per wave) or 8 (the same with 448-row
+ * tiles: four waves of 64 rows and four of 48); pass_slots image slots per pass (0 = all
+ * s_batch). Results do not depend on pass_slots. Forms 0-2 differ from each other only in the
+ * order of GEMM1's FP64 row sums, as do forms 3-8 (6, 7 and 8 equal 4 bit for bit: the same
+ * MFMA sequence per output element); forms 3-8 add each lane's
  * row-sum products in FP32 and form gamma / delta in an FP64 product (agreement with forms
  * 0-2: ~1e-7 relative). */
 int tdscha_ensemble_set_tuning(tdscha_ensemble *ens, int bf16_tiles, int pass_slots);

@@ -675,7 +675,7 @@ void launch_weights_bf16(const Ensemble &e, int sb, int n_ct, int nq, int what, 
 
 void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
   if (e.bf_variant >= 3)
-    launch_gemm1_bf16x3_rp(e, sb, what, e.bf_variant == 3 ? 2 : 1, st);   // 4-7: 96 columns
+    launch_gemm1_bf16x3_rp(e, sb, what, e.bf_variant == 3 ? 2 : 1, st);   // 4-8: 96 columns
   else if (e.bf_variant == 2)
     gemm1_launch_dma(e, sb, what, st);
   else if (e.bf_variant == 1)
@@ -687,7 +687,9 @@ void launch_gemm1_bf16x3(const Ensemble &e, int sb, int what, hipStream_t st) {
 void launch_gemm2_bf16x3(const Ensemble &e, int sb, hipStream_t st) {
   // forms 3-5 write B_s in FP32 for the fused back-rotation; the one-sided rotations (blocks
   // wider than the group tile) read the FP64 B, which form 0 writes
-  if (e.bf_variant == 7 && e.fused_rot)
+  if (e.bf_variant == 8 && e.fused_rot)
+    launch_gemm2_bf16x3_rp(e, sb, 8, st, 0);                          // 448 x 96, 8 waves
+  else if (e.bf_variant == 7 && e.fused_rot)
     launch_gemm2_bf16x3_rp(e, sb, 8, st, 4);                          // 512 x 96, 8 waves
   else if (e.bf_variant == 6 && e.fused_rot)
     launch_gemm2_bf16x3_rp(e, sb, 1, st, 4);                          // 256 x 96

@@ -556,24 +556,17 @@ __device__ inline void rstore_g8(RStage<RWC> &st, bool two, const GSrc8 &s) {
   if (two) rstore_item(st, 64 + (t >> 3), q, s.wa, s.wb, s.yb, s.xb);
 }
 
+// One wave's share of an eight-wave GEMM2 tile: MB row blocks from row0, the 96 columns
+// from c0, every slice of the formed operand staged by all eight waves together. Every wave
+// of the workgroup runs the same k loop with the same barriers (waves of different MB take
+// different instantiations, with the same barrier sequence).
 template <int MB>
-__global__ __launch_bounds__(kR8Threads, 1) void k_gemm2_bf16x3_r8(Gemm2RArgs g) {
-  constexpr int TM = 128 * MB;
-  __shared__ RStage<RWC> st[2];
-  const int total = g.sb * g.n_mt * g.n_ct;
-  const int id = xcd_tile(total);
-  if (id >= total) return;
-  const int sl = id % g.sb;
-  const int rest = id / g.sb;
-  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
-  const int m0 = mt * TM, c0 = ct * RWC;
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
-  const bool two = w < 4;                        // wave-uniform: waves 0-3 form two items
-  const int row0 = m0 + w * 16 * MB;
+__device__ inline void r8_wave(const Gemm2RArgs &g, RStage<RWC> (&st)[2], int sl, int row0,
+                               int c0, const float *__restrict__ wf, bool two) {
+  const int l = threadIdx.x & 63, lr = l & 15, lg = l >> 4;
   const int mi = clamp_blocks(g.n, row0, MB);
   const int nj = clamp_blocks(g.n, c0, RNB);
   const Halves hv = live_halves(mi, nj);
-  const float *wf = g.Wf + (size_t)sl * 2 * g.Np;
 
   v4f acc[MB][RNB];
 #pragma unroll
@@ -634,6 +627,40 @@ __global__ __launch_bounds__(kR8Threads, 1) void k_gemm2_bf16x3_r8(Gemm2RArgs g)
       }
 }
 
+// Tile of 128 MB rows: eight waves of MB row blocks each.
+template <int MB>
+__global__ __launch_bounds__(kR8Threads, 1) void k_gemm2_bf16x3_r8(Gemm2RArgs g) {
+  constexpr int TM = 128 * MB;
+  __shared__ RStage<RWC> st[2];
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id % g.sb;
+  const int rest = id / g.sb;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int w = threadIdx.x >> 6;
+  r8_wave<MB>(g, st, sl, mt * TM + w * 16 * MB, ct * RWC, g.Wf + (size_t)sl * 2 * g.Np, w < 4);
+}
+
+// Tile of 448 rows: waves 0-3 take four row blocks each, waves 4-7 three (so that 1344
+// padded modes make three tiles exactly; a wave of each kind on every SIMD).
+__global__ __launch_bounds__(kR8Threads, 1) void k_gemm2_bf16x3_r7(Gemm2RArgs g) {
+  constexpr int TM = 448;
+  __shared__ RStage<RWC> st[2];
+  const int total = g.sb * g.n_mt * g.n_ct;
+  const int id = xcd_tile(total);
+  if (id >= total) return;
+  const int sl = id % g.sb;
+  const int rest = id / g.sb;
+  const int mt = rest / g.n_ct, ct = rest - mt * g.n_ct;
+  const int w = threadIdx.x >> 6;
+  const float *wf = g.Wf + (size_t)sl * 2 * g.Np;
+  if (w < 4)
+    r8_wave<4>(g, st, sl, mt * TM + 64 * w, ct * RWC, wf, true);
+  else
+    r8_wave<3>(g, st, sl, mt * TM + 256 + 48 * (w - 4), ct * RWC, wf, false);
+}
+
 }  // namespace
 
 void launch_tile_bf16x3(const double *src, int rows, int cols, uint16_t *dst, hipStream_t st) {
@@ -686,8 +713,10 @@ void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, 
   g.Ah = e.XtT_frag; g.Al = e.XtT_frag + NX;
   g.YTf = e.YT_f; g.XtTf = e.XtT_f; g.Wf = e.Wf; g.B = e.B_f;
   g.Np = e.Np; g.npd = e.npd; g.sb = sb;
+  // wcn 8: eight row waves, 128 mb rows (mb = 4) or 448 rows (mb = 0: waves of four and of
+  // three row blocks)
   const int tn = RWC * (wcn == 8 ? 1 : wcn);
-  const int tm = (wcn == 8 ? 128 : 64) * mb;
+  const int tm = wcn == 8 ? (mb == 0 ? 448 : 128 * mb) : 64 * mb;
   g.n_mt = (e.npd + tm - 1) / tm; g.n_ct = (e.npd + tn - 1) / tn;
   // k slices up to the last configuration, rounded up to the loop's step of two
   g.KT = ((e.N + 2 * TK - 1) / (2 * TK)) * 2;
@@ -695,7 +724,9 @@ void launch_gemm2_bf16x3_rp(const Ensemble &e, int sb, int wcn, hipStream_t st, 
   g.n = e.n;
   const int total = sb * g.n_mt * g.n_ct;
   const dim3 grid(8 * ((total + 7) / 8));
-  if (wcn == 8)
+  if (wcn == 8 && mb == 0)
+    hipLaunchKernelGGL(k_gemm2_bf16x3_r7, grid, dim3(kR8Threads), 0, st, g);
+  else if (wcn == 8)
     hipLaunchKernelGGL((k_gemm2_bf16x3_r8<4>), grid, dim3(kR8Threads), 0, st, g);
   else if (wcn == 2)
     hipLaunchKernelGGL((k_gemm2_bf16x3_rp<2, 3>), grid, dim3(RCfg<2>::NT), 0, st, g);

@@ -745,10 +745,10 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
     ms_out = ms;
     return r;
   };
-  // phase 1: the slot count per pass, with the default tile form (form 7 for BF16x3): at
+  // phase 1: the slot count per pass, with the default tile form (form 8 for BF16x3): at
   // least four passes and 168 images per chain (or all images), best of two
   const bool bf = precision == kBF16x3;
-  e.bf_variant = bf ? 7 : 0;
+  e.bf_variant = bf ? 8 : 0;
   int best = e.s_batch, best_se = 1;
   double best_t = 1e300;
   for (int c = e.s_batch, tried = 0; c >= n_chains && tried < 6 && !rc; c /= 2, ++tried) {
@@ -771,12 +771,11 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
   // passes, in two interleaved rounds, after 4 s of passes that bring the board to its
   // sustained clock under the power limit (the forms' order there can differ from that of
   // short bursts at a higher clock); the faster of the two rounds counts, and another form
-  // replaces the default only when it is faster by more than 1 % (the forms 4, 6 and 7 are
-  // within a few per cent of each other on large ensembles, where form 7 is the fastest at
-  // the power limit; the margin keeps the choice from following the timing noise)
+  // replaces the default only when it is faster by more than 1 % (the margin keeps the choice
+  // from following the timing noise between forms that are within a per cent or two)
   int best_v = e.bf_variant;
   if (bf && !rc) {
-    constexpr int n_variants = 8;
+    constexpr int n_variants = 9;
     e.pass_slots = best;
     double one = 0.0;
     rc = timed(best_se, 1, true, one);
@@ -819,7 +818,7 @@ int tdscha_ensemble_autotune(tdscha_ensemble *h, int precision, int n_chains, in
 }
 
 int tdscha_ensemble_set_tuning(tdscha_ensemble *h, int bf16_tiles, int pass_slots) {
-  if (!h || bf16_tiles < 0 || bf16_tiles > 7 || pass_slots < 0) {
+  if (!h || bf16_tiles < 0 || bf16_tiles > 8 || pass_slots < 0) {
     set_error("bad argument");
     return 1;
   }

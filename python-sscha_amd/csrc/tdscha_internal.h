@@ -131,7 +131,9 @@ struct Ensemble {
                               // 5 = GEMM1 of form 4 with GEMM2 of form 3,
                               // 6 = form 4 with 256 x 96 GEMM2 tiles,
                               // 7 = GEMM1 of form 4 with 512 x 96 GEMM2 tiles of
-                              //     eight row waves sharing the formed operand
+                              //     eight row waves sharing the formed operand,
+                              // 8 = the same with 448-row GEMM2 tiles (waves of four
+                              //     and of three row blocks)
   double *Uc = nullptr;       // [chains x npd x npd]  symmetric seed of d2v
   double *uRc = nullptr;      // [chains x npd]        seed of f_av
   double *Us = nullptr;       // [s_batch x npd x npd] f_ups S^T U S f_ups

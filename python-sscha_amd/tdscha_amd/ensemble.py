@@ -209,8 +209,9 @@ class DeviceEnsemble:
         """BF16x3 contraction kernels (0: 256/192 x 128 tiles, 1: 256 x 256, 2: 256 x 256 with
         direct global->LDS copies, 3: 256/192 x 192 with the A operands in registers, 4: the
         same with 96-column tiles, 5: GEMM1 of 4 with GEMM2 of 3, 6: GEMM1 of 4 with 256 x 96
-        GEMM2 tiles, 7: GEMM1 of 4 with 512 x 96 GEMM2 tiles of eight row waves) and image
-        slots per pass (0 = all), as the autotuner sets them (tdscha_ensemble_set_tuning)."""
+        GEMM2 tiles, 7: GEMM1 of 4 with 512 x 96 GEMM2 tiles of eight row waves, 8: the same
+        with 448-row GEMM2 tiles) and image slots per pass (0 = all), as the autotuner sets
+        them (tdscha_ensemble_set_tuning)."""
         L.check(L.lib().tdscha_ensemble_set_tuning(self._h, int(bf16_tiles), int(pass_slots)),
                 "tdscha_ensemble_set_tuning")
 

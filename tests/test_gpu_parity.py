@@ -99,8 +99,8 @@ def test_blocks_wider_than_rotation_tile(precision):
     R1, Y1 = fixtures.random_perturbation(24, 160)
     tol = {"fp64": TOL64, "fp32": 2e-5, "bf16x3": 2e-4}[precision]
     dev = _dev_ens(ens, precisions=("fp64", precision))
-    for v in ((0, 3, 5, 7) if precision == "bf16x3" else (0,)):
-        # tile forms 3-7 keep their register-operand GEMM1 here; GEMM2 falls back to the form
+    for v in ((0, 3, 5, 7, 8) if precision == "bf16x3" else (0,)):
+        # tile forms 3-8 keep their register-operand GEMM1 here; GEMM2 falls back to the form
         # that writes the FP64 B the one-sided rotations read
         dev.set_tuning(v, 0)
         _check(ens, R1, Y1, 1, dev=dev, tol=tol, precision=precision)
@@ -120,26 +120,27 @@ def test_group_tiled_rotation_widths(precision, max_block, seed):
 
 @pytest.mark.parametrize("n,N,seed", [(150, 300, 31), (300, 520, 32), (45, 256, 33)])
 def test_bf16x3_tile_shapes(n, N, seed):
-    """The eight contraction kernel forms against the oracle and against each other. Forms
+    """The nine contraction kernel forms against the oracle and against each other. Forms
     0-2 share their arithmetic and differ only in the order of GEMM1's FP64 row sums (1e-12);
-    so do forms 3-7 among themselves. Forms 3-7 add each lane's six alpha / beta products in
+    so do forms 3-8 among themselves. Forms 3-8 add each lane's six alpha / beta products in
     FP32 and take gamma / delta from an FP64 product, so they agree with forms 0-2 to the
-    FP32 rounding of those partial sums (1e-6 relative). Forms 6 and 7 run form 4's GEMM1
-    and, in their taller GEMM2 tiles (256 rows of four waves, 512 rows of eight), the same
-    MFMA sequence per output element: form 4 bit for bit. The sizes put row and column tiles
-    across the edges of the modes (a 512-row GEMM2 tile over 150 or 300 modes)."""
+    FP32 rounding of those partial sums (1e-6 relative). Forms 6, 7 and 8 run form 4's GEMM1
+    and, in their taller GEMM2 tiles (256 rows of four waves, 512 rows of eight, 448 rows of
+    eight waves of four and of three row blocks), the same MFMA sequence per output element:
+    form 4 bit for bit. The sizes put row and column tiles across the edges of the modes (a
+    448- or 512-row GEMM2 tile over 150 or 300 modes)."""
     ens = fixtures.random_ensemble(seed, n, N, 4, max_block=8)
     R1, Y1 = fixtures.random_perturbation(seed + 1, n)
     dev = _dev_ens(ens, precisions=("fp64", "bf16x3"))
     out = []
-    for v in (0, 1, 2, 3, 4, 5, 6, 7):
+    for v in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         dev.set_tuning(v, 0)
         assert dev.tuning()[0] == v
         out.append(_check(ens, R1, Y1, 1, dev=dev, tol=2e-4, precision="bf16x3"))
-    for k in (6, 7):
+    for k in (6, 7, 8):
         for a, b in zip(out[k], out[4]):
             assert np.array_equal(a, b), k
-    for ref, group, tol in ((0, (1, 2), 1e-12), (3, (4, 5, 6, 7), 1e-12), (0, (3,), 1e-6)):
+    for ref, group, tol in ((0, (1, 2), 1e-12), (3, (4, 5, 6, 7, 8), 1e-12), (0, (3,), 1e-6)):
         for k in group:
             for a, b in zip(out[ref], out[k]):
                 assert np.max(np.abs(a - b)) <= tol * np.max(np.abs(a)), (ref, k)
