@@ -125,11 +125,12 @@ int check_blocks(int n, int n_blocks, const int *N_deg, const int *const *deg_sp
 int choose_s_batch(const Ensemble &e, int requested) {
   if (requested > 0) return requested < e.N_sym ? requested : e.N_sym;
   // workgroups per symmetry image of the least parallel contraction (GEMM2: n x n output
-  // tiles, 192 x 192 in the BF16x3 forms), aiming at about 8192 workgroups per launch: fewer,
-  // larger launches (measured at 432 atoms: BF16x3 84 -> 168 slots per pass 2.2 % faster,
-  // 336 no faster; FP64 26 -> 104 slots 2.2 % faster)
+  // tiles, 192 x 192 in the BF16x3 forms), aiming at about 16384 workgroups per launch and at
+  // most 256 slots: fewer, larger launches (measured at 432 atoms: BF16x3 84 -> 168 -> 256
+  // slots per pass 2.2 % and 1.1 % faster; FP64 26 -> 104 slots 2.2 % faster)
   long per;
-  if (e.prec_mask & (1 << kBF16x3)) {
+  const bool bf = e.prec_mask & (1 << kBF16x3);
+  if (bf) {
     per = (long)((e.npd + 191) / 192) * ((e.npd + 191) / 192);
   } else {
     const long per1 = e.Np / 128;
@@ -137,18 +138,30 @@ int choose_s_batch(const Ensemble &e, int requested) {
     per = per1 < per2 ? per1 : per2;
   }
   if (per < 1) per = 1;
-  long sb = (8192 + per - 1) / per;
+  long sb = (16384 + per - 1) / per;
+  if (sb > 256) sb = 256;
   // workspace per symmetry: Tw, Ys, B (npd^2 doubles each) + reduced-precision Ys + W + G + V
   // + the BF16x3 row sums and FP32 weights
   const double per_s_bytes = 3.0 * 8.0 * e.npd * (double)e.npd + 4.0 * e.npd * (double)e.npd +
                              8.0 * (4.0 * e.Np + 3.0 * e.npd) +
                              8.0 * 4.0 * e.Np * ((e.npd + kBfTileN - 1) / kBfTileN) +
                              4.0 * 2.0 * e.Np + 4.0 * e.npd * (double)e.npd;
-  const double budget = 16.0e9;
+  const double budget = 24.0e9;
   long cap = (long)(budget / per_s_bytes);
   if (cap < 1) cap = 1;
   if (sb > cap) sb = cap;
   if (sb > e.N_sym) sb = e.N_sym;
+  // BF16x3: whole waves of tile form 8's GEMM2 (one workgroup per CU, long workgroups, so
+  // that a partial last wave leaves CUs idle for a workgroup's whole length): a multiple of
+  // CUs / gcd(CUs, workgroups per slot) slots when there are that many
+  int cus = 0;
+  if (bf && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device) ==
+                hipSuccess && cus > 0) {
+    long wg = (long)((e.npd + 447) / 448) * ((e.npd + 95) / 96), a = cus, b = wg;
+    while (b) { const long t = a % b; a = b; b = t; }
+    const long q = cus / a;
+    if (sb >= q) sb = sb / q * q;
+  }
   if (sb < 1) sb = 1;
   return (int)sb;
 }
