@@ -239,7 +239,7 @@ def main():
                     help="skip the zone-centre (translation-invariant) operator line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+        print("bench.py measures one GPU: this package has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 

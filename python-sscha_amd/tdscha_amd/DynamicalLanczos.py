@@ -61,7 +61,7 @@ class Lanczos(object):
     def __init__(self, ensemble=None, mode=MODE_GPU_FP64, use_wigner=False, device=0,
                  s_batch=0):
         if mode not in _GPU_PRECISION:
-            raise ValueError("mode %r: this build implements the device modes MODE_GPU_FP64, "
+            raise ValueError("mode %r: this package implements the device modes MODE_GPU_FP64, "
                              "MODE_GPU_FP32 and MODE_GPU_BF16X3" % (mode,))
         self.mode = mode
         self.device = device
@@ -657,10 +657,10 @@ class Lanczos(object):
     def build_lanczos_matrix_from_coeffs(self, use_arnoldi=False):
         """The tridiagonal matrix of the run (Py:4571-4612): a on the diagonal, b below and c
         above (b above too for a symmetric run without c), mapped back through reverse_L and
-        shift_value, so that it represents L itself. This build keeps no Arnoldi matrix:
+        shift_value, so that it represents L itself. This package keeps no Arnoldi matrix:
         use_arnoldi=True raises."""
         if use_arnoldi:
-            raise NotImplementedError("no Arnoldi matrix is kept by this build's Lanczos")
+            raise NotImplementedError("no Arnoldi matrix is kept by this Lanczos")
         N = len(self.a_coeffs)
         M = np.zeros((N, N), dtype=TYPE_DP)
         for i in range(N):

@@ -1,7 +1,7 @@
 """Write the committed golden fixtures from the reference tree's own test data.
 
-Run in the build container (it reads /root/reference, which does not exist on
-the GPU box):  python tests/golden/make_fixtures.py
+Run where the reference tree is present (it reads /root/reference; the GPU tests only
+read the fixtures it writes):  python tests/golden/make_fixtures.py
 
 Inputs (all data files shipped in the reference's tests, loaded with
 allow_pickle=False / text readers):

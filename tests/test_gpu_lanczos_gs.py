@@ -7,7 +7,7 @@ Tolerances. Where the Krylov space stays well conditioned (the reference's own s
 on its SnTe ensemble, 11 steps until the space is exhausted) the coefficients agree to 1e-10.
 From a random start the biconjugate coefficients are ill conditioned: in the oracle, 1e-15
 relative noise per L-application moves a_k by up to 3e-7 within 16 steps even with the
-projection (p_k . q_k falls to ~1e-7, DESIGN.md records the measurement), so there the bar
+projection (p_k . q_k falls to ~1e-7), so there the bar
 is biorthogonality of the stored basis (1e-12), the recurrence run on the device operator
 by the oracle (1e-7 over the first 10 steps, 1e-4 over 30, per kind of the largest
 coefficient), and the spectrum of a complete Krylov space (eigenvalues of L to 1e-10 /
